@@ -1,0 +1,244 @@
+"""Benchmark: one row partition of the distributed SpMM C = A·B on one MI355X.
+
+Workload (the real shape of one partition): R-MAT graph with 2^22 nodes,
+average degree 32, a = 0.57, b = c = 0.19, symmetrised, self-loops and
+duplicates dropped, GCN-normalised values, Cuthill-McKee locality order;
+8 row partitions of 2^19 rows, partition 0 (the heaviest one, which sets a
+distributed step's critical path); B is 2^22 x 128 FP32 uniform[-1, 1],
+resident in HBM.  One step = gather-pack of the unique remote B rows + the
+degree-binned row kernel + the hub reduction, C overwritten.
+
+value = algorithmic bytes of the partition SpMM / step time, with
+  bytes_alg = 4(M+1) + nnz_p*(4+4) + (U_l + U_r)*N*s_B + M*N*4
+(each distinct B row read once, C written once).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "SpMM HBM GB/s (% of peak) FP32 vs BF16 at stated error; served req/s + p99"
+
+
+def bytes_alg(M, nnz, U, N, s_B, s_sc=0):
+    return 4 * (M + 1) + nnz * 8 + U * (N * s_B + s_sc) + M * N * 4
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--log-n", type=int, default=22)
+    ap.add_argument("--degree", type=float, default=32)
+    ap.add_argument("--nparts", type=int, default=8)
+    ap.add_argument("--rank", type=int, default=0)
+    ap.add_argument("--dim", type=int, default=128)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one partition on one GPU; runs on more than one GPU are not part of "
+              "this build (the multi-GPU exchange is design-only).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import cola_amd as cola
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t0 = time.time()
+    g = cola.Graph.rmat(args.log_n, args.degree, seed=2, reorder=True)
+    n, nnz_total, _ = g.info()
+    part = g.partition(args.nparts, args.rank)
+    info = part.info()
+    P, M, N = info["nodesPerPE"], info["num_rows"], args.dim
+    plan = cola.Plan(part)
+    nnz_p = plan.nnz
+    U_l, U_r = plan.U_l, plan.U_r
+    log(f"graph n={n} nnz={nnz_total} partition {args.rank}/{args.nparts}: M={M} nnz_p={nnz_p} "
+        f"U_l={U_l} U_r={U_r} items={plan.info['num_items']} hubs={plan.info['num_hub_rows']} "
+        f"setup {time.time() - t0:.1f}s")
+
+    gen = torch.Generator(device=dev).manual_seed(1234)
+    B = torch.rand((P * args.nparts, N), generator=gen, device=dev, dtype=torch.float32).mul_(2).sub_(1)
+    B_local = B[args.rank * P:(args.rank + 1) * P]
+    packed = torch.empty((max(U_r, 1), N), dtype=torch.float32, device=dev)
+    C = torch.empty((P, N), dtype=torch.float32, device=dev)
+
+    def step():
+        plan.pack(B, packed)
+        plan.spmm(C, B_local, packed, flags=cola.OVERWRITE)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for k in range(args.steps):
+        ev[k][0].record()
+        plan.pack(B, packed)
+        ev[k][1].record()
+        plan.spmm(C, B_local, packed, flags=cola.OVERWRITE)
+        ev[k][2].record()
+    end.record()
+    torch.cuda.synchronize()
+    total_ms = start.elapsed_time(end)
+    ms = total_ms / args.steps
+    pack_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
+    spmm_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
+    balg = bytes_alg(M, nnz_p, U_l + U_r, N, 4)
+    value = balg / (ms * 1e-3) / 1e9
+    # Row kernel's own algorithmic bytes: entry lists + work items + distinct
+    # local B rows + packed rows + C.
+    kbytes = nnz_p * 8 + plan.info["num_items"] * 16 + (U_l + U_r) * N * 4 + M * N * 4
+    k_gbs = kbytes / (spmm_ms * 1e-3) / 1e9
+    log(f"step {ms:.3f} ms (pack {pack_ms:.3f} + spmm {spmm_ms:.3f}); {value:.0f} GB/s alg; "
+        f"spmm kernel {k_gbs:.0f} GB/s; gather {nnz_p * N * 4 / (spmm_ms * 1e-3) / 1e9:.0f} GB/s")
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "plan_rows_f32_rmat22_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    extras = {}
+    if not args.no_extras:
+        # Correctness of this very run on a sample of rows (FP64 oracle).
+        from oracle import oracle as orc
+        rp, ci, va = g.arrays()
+        Bh = B.cpu().numpy()
+        Ch = C.cpu().numpy()
+        sample = np.arange(0, M, max(1, M // 512))
+        ref = np.stack([orc.spmm_f64(rp, ci, va, args.rank * P + int(i), args.rank * P + int(i) + 1, Bh[:n])[0]
+                        for i in sample])
+        err = float(np.linalg.norm(Ch[sample] - ref) / np.linalg.norm(ref))
+        extras["fp32_rel_fro_sample"] = err
+        log(f"fp32 sample rel_fro {err:.2e}")
+        # BF16 storage of B (FP32 accumulation, Kahan on hub rows).
+        Bq, _ = cola.quantize_rows(B, cola.BF16)
+        pq = torch.empty((max(U_r, 1), N), dtype=torch.uint16, device=dev)
+        Cq = torch.empty_like(C)
+
+        def step_bf16():
+            plan.pack(Bq, pq, dtype=cola.BF16)
+            plan.spmm(Cq, Bq[args.rank * P:(args.rank + 1) * P], pq, dtype=cola.BF16,
+                      flags=cola.OVERWRITE | cola.KAHAN)
+
+        for _ in range(args.warmup):
+            step_bf16()
+        torch.cuda.synchronize()
+        s2, e2 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s2.record()
+        for _ in range(args.steps):
+            step_bf16()
+        e2.record()
+        torch.cuda.synchronize()
+        ms_bf16 = s2.elapsed_time(e2) / args.steps
+        Cqh = Cq.cpu().numpy()
+        err_bf16 = float(np.linalg.norm(Cqh[sample] - ref) / np.linalg.norm(ref))
+        extras["bf16"] = {"ms_per_step": ms_bf16,
+                          "GBps_alg": bytes_alg(M, nnz_p, U_l + U_r, N, 2) / (ms_bf16 * 1e-3) / 1e9,
+                          "speedup_vs_fp32": ms / ms_bf16, "rel_fro_sample_vs_fp64": err_bf16}
+        log(f"bf16 step {ms_bf16:.3f} ms ({ms / ms_bf16:.2f}x), rel_fro {err_bf16:.2e}")
+        del Bq, pq, Cq
+        # Reference-semantics path: nnz_split virtual rows + CSC scatter atomics.
+        a = part.arrays()
+        tl, pl, _ = cola.nnz_split(a["csr_rowIdx_l"], a["csr_rowPtr_l"], len(a["csr_colIdx_l"]))
+        tr, pr, _ = cola.nnz_split(a["csc_colIdx_r"], a["csc_colPtr_r"], len(a["csc_rowIdx_r"]))
+        zl = cola.d_csr_zip(pl, a["csr_colIdx_l"], tl, a["csr_val_l"])
+        zr = cola.d_csc_zip(pr, a["csc_rowIdx_r"], tr, a["csc_val_r"])
+        Cr = torch.zeros_like(C)
+        for _ in range(2):
+            cola.dist_spmm_ppl_batch(Cr, B, packed, zl, zr, N, P, args.rank)
+        torch.cuda.synchronize()
+        s3, e3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = max(3, args.steps // 4)
+        s3.record()
+        for _ in range(reps):
+            cola.dist_spmm_ppl_batch(Cr, B, packed, zl, zr, N, P, args.rank)
+        e3.record()
+        torch.cuda.synchronize()
+        ms_ref = s3.elapsed_time(e3) / reps
+        extras["reference_semantics_path"] = {"ms_per_step": ms_ref,
+                                              "GBps_alg": balg / (ms_ref * 1e-3) / 1e9}
+        log(f"reference-semantics path (atomics) {ms_ref:.3f} ms")
+        del Cr, zl, zr
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+        rp, ci, va = g.arrays()
+        Bh = B.cpu().numpy()[:n]
+        lb = args.rank * P
+        # Bounded sample: whole row blocks of the partition until ~cpu-seconds.
+        rows_per_block = max(1, M // 16)
+        done_rows = 0
+        done_nnz = 0
+        t_cpu = 0.0
+        threads = 1
+        Cc = np.empty((rows_per_block, N), np.float32)
+        blk = 0
+        while t_cpu < args.cpu_seconds and done_rows < M:
+            r0 = lb + blk * rows_per_block
+            r1 = min(lb + M, r0 + rows_per_block)
+            tt = time.perf_counter()
+            _, threads = orc.spmm_omp_f32(rp, ci, va, r0, r1, Bh, Cc[:r1 - r0])
+            t_cpu += time.perf_counter() - tt
+            done_rows += r1 - r0
+            done_nnz += int(rp[r1] - rp[r0])
+            blk += 1
+        # Scale the algorithmic bytes to the sampled rows.
+        frac_rows = done_rows / M
+        cb = 4 * (done_rows + 1) + done_nnz * 8 + int((U_l + U_r) * frac_rows) * N * 4 + done_rows * N * 4
+        cpu = {"value": cb / t_cpu / 1e9, "unit": "GB/s", "cores": int(threads), "kind": "port",
+               "sample": f"OpenMP FP32 CSR SpMM (oracle), rows [{lb},{lb + done_rows}) of partition "
+                         f"{args.rank} ({done_nnz} nnz, {done_rows / M:.0%} of its rows), {t_cpu:.1f}s; "
+                         f"bytes scaled by the sampled row fraction"}
+        log(f"cpu baseline {cpu['value']:.1f} GB/s on {threads} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded R-MAT graph, uniform[-1,1] B generated on the GPU)",
+        "config": {"workload": f"rmat{args.log_n}_deg{int(args.degree)}_cm_part{args.rank}of{args.nparts}"
+                               f"_dim{N}_f32",
+                   "nodes": n, "nnz_graph": nnz_total, "rows": M, "nnz_partition": nnz_p,
+                   "unique_local_cols": U_l, "unique_remote_cols": U_r, "dim": N,
+                   "bytes_alg": balg, "step": "pack + plan row kernel + hub reduce, C overwritten"},
+        "roofline": {"bound": "hbm", "achieved": round(k_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "plan_rows_kernel (+ hub_reduce_kernel)", "kernel_ms": round(spmm_ms, 4),
+                     "kernel_bytes_alg": kbytes},
+        "cpu_baseline": cpu,
+        "kernels_ms": {"pack": round(pack_ms, 4), "spmm": round(spmm_ms, 4)},
+        "extras": extras,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

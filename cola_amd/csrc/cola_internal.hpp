@@ -1,0 +1,70 @@
+// Internal structures shared by the host library (host.cpp) and the device
+// side (spmm.hip).  Not part of the public ABI.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/cola_spmm.h"
+
+namespace cola {
+
+// Records the failure text for cola_last_error() and returns `code`.
+cola_status fail(cola_status code, const std::string &msg);
+
+struct Graph {
+  int64_t n = 0;
+  std::vector<int64_t> row_ptr;  // n+1
+  std::vector<int32_t> col_idx;  // nnz, sorted and unique within each row
+  std::vector<float> val;        // nnz
+};
+
+struct Partition {
+  int64_t num_nodes = 0;
+  int32_t nGPU = 0, rank = 0, nodesPerPE = 0, num_rows = 0;
+  // prep_torch_par.py:24-79 layout.
+  std::vector<int32_t> csr_rowPtr_l, csr_colIdx_l, csr_rowIdx_l;
+  std::vector<float> csr_val_l;
+  std::vector<int32_t> csc_colPtr_r, csc_rowIdx_r, csc_colIdx_r;
+  std::vector<float> csc_val_r;
+};
+
+// Validates the structural invariants of a partition (what the reference's
+// writer asserts, prep_torch_par.py:50,53,71,74, plus index ranges).
+cola_status validate_partition(const Partition &p);
+
+// Host form of the degree-binned plan (see cola_plan_create).
+struct WorkItem {  // 16 bytes, loaded as one int4 by the row kernel
+  int32_t out;     // >= 0: output row; < 0: -(hub chunk + 1)
+  int32_t begin;   // first entry in the merged entry list
+  int32_t end;     // one past the last entry
+  int32_t flags;   // bit 0: compensated (Kahan) summation allowed
+};
+
+struct HostPlan {
+  int32_t num_rows = 0, nodesPerPE = 0, U_r = 0, U_l = 0;
+  int32_t max_dim = 0;
+  int64_t nnz = 0;
+  // Merged row entries: col >= 0 -> local column of B_p; col < 0 -> packed
+  // remote row ~col.
+  std::vector<int32_t> cols;
+  std::vector<float> vals;
+  std::vector<WorkItem> items;       // non-empty rows and hub chunks
+  std::vector<int32_t> empty_rows;   // rows without entries (OVERWRITE)
+  std::vector<int32_t> xcd_begin;    // 9 entries: item range per XCD
+  std::vector<int32_t> hub_rows;     // output row of each hub
+  std::vector<int32_t> hub_chunk_ptr;// chunk range per hub (num_hubs+1)
+  std::vector<int32_t> remote_cols;  // unique remote global columns (U_r)
+};
+
+cola_status build_plan(const Partition &p, const cola_plan_opts &opts,
+                       HostPlan *out);
+
+}  // namespace cola
+
+struct cola_graph_s {
+  cola::Graph g;
+};
+struct cola_partition_s {
+  cola::Partition p;
+};

@@ -1,0 +1,1038 @@
+// Device side of the single-partition SpMM engine for MI355X (gfx950, CDNA4,
+// wave64).  Kernels:
+//   pack_rows_kernel      gather-pack of unique remote B rows into one
+//                         contiguous buffer (replaces the per-warp NVSHMEM get,
+//                         dist_spmm_ppl_batch.cuh:120-125, and
+//                         batching_data_ppl, cuh:58-73)
+//   ppl0_kernel           reference-semantics partition SpMM: virtual local
+//                         CSR rows + virtual remote CSC columns, FP32 atomics
+//                         into C (ppl0_batch_SpMM_kernel, cuh:76-200)
+//   plan_rows_kernel      degree-binned, atomic-free row kernel over the
+//                         merged (local + packed-remote) row lists
+//   hub_reduce_kernel     fixed-order reduction of hub-row chunk partials
+//   zero_rows_kernel      zeroes rows without entries (overwrite mode)
+//   quantize_rows_kernel  FP32 rows -> BF16 / FP16 / FP8-e4m3 (+ row scale)
+//   error_norms_kernel    FP64 error norms of C against an FP64 reference
+//
+// SpMM is HBM/cache-bandwidth bound (about 2 FLOP per 8+ bytes), so nothing
+// here uses MFMA.  The dense dimension is spread across the lanes of a row
+// group (G lanes, 4 consecutive elements per lane per vector), several row
+// groups share a wave, and (column, value) pairs are loaded once per 64
+// entries with one coalesced load and broadcast to the groups with
+// ds_bpermute.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "cola_internal.hpp"
+
+using cola::fail;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Element formats.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float bf16_to_f32(uint32_t h) {
+  return __uint_as_float(h << 16);
+}
+__device__ __forceinline__ float f16_to_f32(uint32_t h) {
+  return __half2float(__ushort_as_half((unsigned short)h));
+}
+// OCP FP8 E4M3 (e4m3fn): 1 sign, 4 exponent (bias 7), 3 mantissa bits; code
+// with exponent 0 is subnormal (m * 2^-9); 0x7f/0xff are NaN (never written).
+__device__ __forceinline__ float fp8e4m3_to_f32(uint32_t c) {
+  const uint32_t e = (c >> 3) & 15u, m = c & 7u;
+  const float mag = e == 0 ? (float)m * 0x1p-9f
+                           : __uint_as_float(((c & 0x7fu) << 20) + (120u << 23));
+  return (c & 0x80u) ? -mag : mag;
+}
+__device__ __forceinline__ uint32_t f32_to_bf16_rne(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return ((u >> 16) & 0x8000u) | 0x7fc0u;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+// Round-to-nearest-even into e4m3fn, saturating at +-448 (finite inputs).
+__device__ __forceinline__ uint32_t f32_to_fp8e4m3_rne(float f) {
+  const uint32_t sign = (__float_as_uint(f) >> 24) & 0x80u;
+  float a = fabsf(f);
+  if (!(a < 448.0f)) return sign | 0x7eu;  // also maps NaN to max finite
+  if (a < 0x1p-6f) {                      // subnormal range: quantum 2^-9
+    const uint32_t q = (uint32_t)__builtin_rintf(a * 512.0f);  // 0..8
+    return sign | q;                                            // 8 == min normal
+  }
+  uint32_t u = __float_as_uint(a);
+  u += 0x7ffffu + ((u >> 20) & 1u);  // RNE from 23 to 3 mantissa bits
+  const int e = (int)(u >> 23) - 127 + 7;
+  const uint32_t m = (u >> 20) & 7u;
+  if (e > 15 || (e == 15 && m == 7)) return sign | 0x7eu;
+  return sign | ((uint32_t)e << 3) | m;
+}
+
+// Loads EPL consecutive elements (EPL = 4: one 16/8/4-byte vector; EPL = 1:
+// one scalar) of a row and widens them to FP32.
+template <int DT, int EPL>
+struct Row;
+template <>
+struct Row<COLA_F32, 4> {
+  using T = float;
+  static __device__ __forceinline__ void load(const T *p, float (&x)[4]) {
+    const float4 v = *reinterpret_cast<const float4 *>(p);
+    x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+  }
+};
+template <>
+struct Row<COLA_F32, 1> {
+  using T = float;
+  static __device__ __forceinline__ void load(const T *p, float (&x)[1]) { x[0] = *p; }
+};
+template <>
+struct Row<COLA_BF16, 4> {
+  using T = uint16_t;
+  static __device__ __forceinline__ void load(const T *p, float (&x)[4]) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(p);
+    x[0] = bf16_to_f32(v.x & 0xffffu); x[1] = bf16_to_f32(v.x >> 16);
+    x[2] = bf16_to_f32(v.y & 0xffffu); x[3] = bf16_to_f32(v.y >> 16);
+  }
+};
+template <>
+struct Row<COLA_BF16, 1> {
+  using T = uint16_t;
+  static __device__ __forceinline__ void load(const T *p, float (&x)[1]) { x[0] = bf16_to_f32(*p); }
+};
+template <>
+struct Row<COLA_F16, 4> {
+  using T = uint16_t;
+  static __device__ __forceinline__ void load(const T *p, float (&x)[4]) {
+    const uint2 v = *reinterpret_cast<const uint2 *>(p);
+    x[0] = f16_to_f32(v.x & 0xffffu); x[1] = f16_to_f32(v.x >> 16);
+    x[2] = f16_to_f32(v.y & 0xffffu); x[3] = f16_to_f32(v.y >> 16);
+  }
+};
+template <>
+struct Row<COLA_F16, 1> {
+  using T = uint16_t;
+  static __device__ __forceinline__ void load(const T *p, float (&x)[1]) { x[0] = f16_to_f32(*p); }
+};
+template <>
+struct Row<COLA_FP8_E4M3, 4> {
+  using T = uint8_t;
+  static __device__ __forceinline__ void load(const T *p, float (&x)[4]) {
+    const uint32_t v = *reinterpret_cast<const uint32_t *>(p);
+    x[0] = fp8e4m3_to_f32(v & 0xffu); x[1] = fp8e4m3_to_f32((v >> 8) & 0xffu);
+    x[2] = fp8e4m3_to_f32((v >> 16) & 0xffu); x[3] = fp8e4m3_to_f32(v >> 24);
+  }
+};
+template <>
+struct Row<COLA_FP8_E4M3, 1> {
+  using T = uint8_t;
+  static __device__ __forceinline__ void load(const T *p, float (&x)[1]) { x[0] = fp8e4m3_to_f32(*p); }
+};
+
+size_t dtype_bytes(int dt) {
+  switch (dt) {
+    case COLA_F32: return 4;
+    case COLA_BF16: case COLA_F16: return 2;
+    case COLA_FP8_E4M3: return 1;
+  }
+  return 0;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ---------------------------------------------------------------------------
+// Gather-pack: packed[u] = B[cols[u]].  A group of lanes copies one row with
+// 16-byte (or, when unaligned, element-sized) accesses.
+// ---------------------------------------------------------------------------
+template <typename V>
+__global__ __launch_bounds__(256) void pack_rows_kernel(
+    const char *__restrict__ B, int64_t ldB_bytes, const int32_t *__restrict__ cols,
+    int64_t num_rows, int64_t row_vecs, char *__restrict__ out,
+    int64_t ldO_bytes, const float *__restrict__ scales, float *__restrict__ out_scales,
+    int lanes_per_row) {
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t groups_total = (int64_t)gridDim.x * blockDim.x / lanes_per_row;
+  const int64_t g0 = tid / lanes_per_row;
+  const int gl = (int)(tid % lanes_per_row);
+  for (int64_t r = g0; r < num_rows; r += groups_total) {
+    const int64_t src = cols[r];
+    const V *s = reinterpret_cast<const V *>(B + src * ldB_bytes);
+    V *d = reinterpret_cast<V *>(out + r * ldO_bytes);
+    for (int64_t k = gl; k < row_vecs; k += lanes_per_row) d[k] = s[k];
+    if (out_scales && gl == 0) out_scales[r] = scales[src];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Reference-semantics kernel (ppl0_batch_SpMM_kernel, cuh:76-200).  One group
+// of G lanes per task: tasks [0, L) are virtual local rows, tasks [L, L+R)
+// are virtual remote columns (visited in reverse when peid & 2, cuh:115-118).
+// ---------------------------------------------------------------------------
+template <int EPL, int G, int NV>
+__global__ __launch_bounds__(256) void ppl0_kernel(
+    float *__restrict__ out, const float *__restrict__ in, const float *__restrict__ packed,
+    const int32_t *__restrict__ rowPtr_l, const int32_t *__restrict__ colIdx_l,
+    const int32_t *__restrict__ rowIdx_l, const float *__restrict__ val_l,
+    const int32_t *__restrict__ colPtr_r, const int32_t *__restrict__ rowIdx_r,
+    const int32_t *__restrict__ packIdx_r, const float *__restrict__ val_r,
+    int local_lb, int remote_lb, int L, int R, int dim, int peid) {
+  using RowT = Row<COLA_F32, EPL>;
+  const int lane = threadIdx.x & 63;
+  const int sub = lane / G, gl = lane % G;
+  const int64_t T = (int64_t)L + R;
+  const int64_t task = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  if (task >= T) return;
+  const int gbase = sub * G;
+  float acc[NV][EPL];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) acc[k][q] = 0.f;
+
+  if (task < L) {
+    const int v = local_lb + (int)task;
+    const int b = rowPtr_l[v], e = rowPtr_l[v + 1];
+    for (int base = b; base < e; base += G) {
+      const int cnt = min(G, e - base);
+      int mc = 0;
+      float mv = 0.f;
+      if (gl < cnt) {
+        mc = colIdx_l[base + gl];
+        mv = val_l[base + gl];
+      }
+      for (int j = 0; j < cnt; j += 4) {
+        float x[4][NV][EPL];
+        float a[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int jj = min(j + u, cnt - 1);
+          const int c = __shfl(mc, gbase + jj);
+          a[u] = (j + u < cnt) ? __shfl(mv, gbase + jj) : 0.f;
+          const float *rp = in + (int64_t)c * dim;
+#pragma unroll
+          for (int k = 0; k < NV; ++k) {
+            const int col = (k * G + gl) * EPL;
+            if (col < dim) RowT::load(rp + col, x[u][k]);
+            else
+#pragma unroll
+              for (int q = 0; q < EPL; ++q) x[u][k][q] = 0.f;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+#pragma unroll
+            for (int q = 0; q < EPL; ++q) acc[k][q] += a[u] * x[u][k][q];
+      }
+    }
+    const int row = rowIdx_l[v];
+    float *op = out + (int64_t)row * dim;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const int col = (k * G + gl) * EPL;
+      if (col < dim)
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) unsafeAtomicAdd(op + col + q, acc[k][q]);
+    }
+    return;
+  }
+
+  // Remote virtual column: packed row held in registers, scattered with atomics.
+  const int t = (int)(task - L);
+  const int idx = remote_lb + ((peid & 2) == 0 ? t : (int)(R - t - 1));
+  const float *rp = packed + (int64_t)packIdx_r[idx] * dim;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const int col = (k * G + gl) * EPL;
+    if (col < dim) RowT::load(rp + col, acc[k]);
+  }
+  const int b = colPtr_r[idx], e = colPtr_r[idx + 1];
+  for (int base = b; base < e; base += G) {
+    const int cnt = min(G, e - base);
+    int mr = 0;
+    float mv = 0.f;
+    if (gl < cnt) {
+      mr = rowIdx_r[base + gl];
+      mv = val_r[base + gl];
+    }
+    for (int j = 0; j < cnt; ++j) {
+      const int r = __shfl(mr, gbase + j);
+      const float a = __shfl(mv, gbase + j);
+      float *op = out + (int64_t)r * dim;
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const int col = (k * G + gl) * EPL;
+        if (col < dim)
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) unsafeAtomicAdd(op + col + q, a * acc[k][q]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Plan row kernel.  Each wave takes work items (one row, or one chunk of a hub
+// row); its 64/G row groups walk the item's entries with stride 64/G, U loads
+// in flight per group, then the groups' sums are combined across the wave.
+// Blocks are assigned to XCDs round-robin by the dispatcher (block b runs on
+// XCD b % 8), so block b walks XCD (b % 8)'s contiguous item range and rows
+// that share B rows share that XCD's L2.
+// ---------------------------------------------------------------------------
+template <int DT, int EPL, int G, int NV, int U>
+__global__ __launch_bounds__(256) void plan_rows_kernel(
+    const int4 *__restrict__ items, const int32_t *__restrict__ xcd_begin,
+    const int32_t *__restrict__ cols, const float *__restrict__ vals,
+    const typename Row<DT, EPL>::T *__restrict__ Bl, int64_t ldB,
+    const float *__restrict__ sl, const typename Row<DT, EPL>::T *__restrict__ Bp,
+    int64_t ldP, const float *__restrict__ sp, float *__restrict__ C, int64_t ldC,
+    float *__restrict__ partials, int pstride, int dim, int accumulate, int kahan) {
+  using RowT = Row<DT, EPL>;
+  constexpr int S = 64 / G;
+  const int lane = threadIdx.x & 63;
+  const int sub = lane / G, gl = lane % G;
+  const int wpb = blockDim.x >> 6;
+  const int wid = uniform(threadIdx.x >> 6);
+  const int xcd = blockIdx.x & 7;
+  const int nblk = (gridDim.x - xcd + 7) >> 3;
+  const int wave = (blockIdx.x >> 3) * wpb + wid;
+  const int nwave = nblk * wpb;
+  const int ib = xcd_begin[xcd], ie = xcd_begin[xcd + 1];
+
+  for (int it = ib + wave; it < ie; it += nwave) {
+    const int4 item = items[it];
+    const int b = item.y, e = item.z;
+    const bool comp = kahan && (item.w & 1);
+    float acc[NV][EPL], err[NV][EPL];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[k][q] = err[k][q] = 0.f;
+
+    for (int base = b; base < e; base += 64) {
+      const int n = min(64, e - base);
+      int mc = 0;
+      float mv = 0.f;
+      if (lane < n) {
+        mc = cols[base + lane];
+        mv = vals[base + lane];
+      }
+      for (int j = 0; j < n; j += S * U) {
+        float x[U][NV][EPL];
+        float a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int ei = j + u * S + sub;
+          const bool ok = ei < n;
+          const int src = (ok ? ei : 0) << 2;
+          const int c = __builtin_amdgcn_ds_bpermute(src, mc);
+          float av = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv)));
+          const typename RowT::T *rp = c >= 0 ? Bl + (int64_t)c * ldB : Bp + (int64_t)(~c) * ldP;
+          if constexpr (DT == COLA_FP8_E4M3) {
+            if (ok) av *= (c >= 0 ? sl[c] : sp[~c]);
+          }
+          a[u] = ok ? av : 0.f;
+#pragma unroll
+          for (int k = 0; k < NV; ++k) {
+            const int col = (k * G + gl) * EPL;
+            if (ok && col < dim) RowT::load(rp + col, x[u][k]);
+            else
+#pragma unroll
+              for (int q = 0; q < EPL; ++q) x[u][k][q] = 0.f;
+          }
+        }
+        if (!comp) {
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+#pragma unroll
+              for (int q = 0; q < EPL; ++q) acc[k][q] = fmaf(a[u], x[u][k][q], acc[k][q]);
+        } else {
+          // Kahan: err carries the low-order part lost by acc.
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+#pragma unroll
+              for (int q = 0; q < EPL; ++q) {
+                const float y = fmaf(a[u], x[u][k][q], -err[k][q]);
+                const float t = acc[k][q] + y;
+                err[k][q] = (t - acc[k][q]) - y;
+                acc[k][q] = t;
+              }
+        }
+      }
+    }
+    // Combine the S row groups of the wave.
+#pragma unroll
+    for (int off = G; off < 64; off <<= 1)
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) {
+          if (!comp) {
+            acc[k][q] += __shfl_xor(acc[k][q], off);
+          } else {
+            // Neumaier combination of (acc, -err) pairs.
+            const float s2 = __shfl_xor(acc[k][q], off);
+            const float e2 = __shfl_xor(err[k][q], off);
+            const float s1 = acc[k][q];
+            const float t = s1 + s2;
+            const float lost = fabsf(s1) >= fabsf(s2) ? (s1 - t) + s2 : (s2 - t) + s1;
+            acc[k][q] = t;
+            err[k][q] = err[k][q] + e2 - lost;
+          }
+        }
+    if (sub == 0) {
+      float *dst;
+      bool acc_old;
+      if (item.x >= 0) {
+        dst = C + (int64_t)item.x * ldC;
+        acc_old = accumulate != 0;
+      } else {
+        dst = partials + (int64_t)(-item.x - 1) * pstride;
+        acc_old = false;
+      }
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const int col = (k * G + gl) * EPL;
+        if (col < dim) {
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) {
+            float v = comp ? acc[k][q] - err[k][q] : acc[k][q];
+            if (acc_old) v += dst[col + q];
+            dst[col + q] = v;
+          }
+        }
+      }
+    }
+  }
+}
+
+// One wave per hub row: partial sums of its chunks added in chunk order.
+__global__ __launch_bounds__(256) void hub_reduce_kernel(
+    const int32_t *__restrict__ hub_rows, const int32_t *__restrict__ hub_ptr, int num_hubs,
+    const float *__restrict__ partials, int pstride, float *__restrict__ C, int64_t ldC,
+    int dim, int accumulate, int kahan) {
+  const int h = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (h >= num_hubs) return;
+  const int row = hub_rows[h], c0 = hub_ptr[h], c1 = hub_ptr[h + 1];
+  float *dst = C + (int64_t)row * ldC;
+  for (int col = lane; col < dim; col += 64) {
+    float s = 0.f, err = 0.f;
+    for (int c = c0; c < c1; ++c) {
+      const float x = partials[(int64_t)c * pstride + col];
+      if (kahan) {
+        const float y = x - err;
+        const float t = s + y;
+        err = (t - s) - y;
+        s = t;
+      } else {
+        s += x;
+      }
+    }
+    dst[col] = accumulate ? dst[col] + s : s;
+  }
+}
+
+__global__ __launch_bounds__(256) void zero_rows_kernel(const int32_t *__restrict__ rows,
+                                                        int num, float *__restrict__ C,
+                                                        int64_t ldC, int dim) {
+  const int r = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= num) return;
+  float *dst = C + (int64_t)rows[r] * ldC;
+  for (int col = lane; col < dim; col += 64) dst[col] = 0.f;
+}
+
+// One wave per row.  FP8 rows: scale = max|x| / 448 (1 for an all-zero row).
+__global__ __launch_bounds__(256) void quantize_rows_kernel(const float *__restrict__ src,
+                                                            int64_t lds, int64_t rows, int dim,
+                                                            int dt, void *__restrict__ dst,
+                                                            int64_t ldd, float *__restrict__ scales) {
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (r >= rows) return;
+  const float *s = src + r * lds;
+  if (dt == COLA_F32) {
+    float *d = (float *)dst + r * ldd;
+    for (int c = lane; c < dim; c += 64) d[c] = s[c];
+  } else if (dt == COLA_BF16) {
+    uint16_t *d = (uint16_t *)dst + r * ldd;
+    for (int c = lane; c < dim; c += 64) d[c] = (uint16_t)f32_to_bf16_rne(s[c]);
+  } else if (dt == COLA_F16) {
+    uint16_t *d = (uint16_t *)dst + r * ldd;
+    for (int c = lane; c < dim; c += 64) d[c] = __half_as_ushort(__float2half(s[c]));
+  } else {
+    float m = 0.f;
+    for (int c = lane; c < dim; c += 64) m = fmaxf(m, fabsf(s[c]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    const float scale = m > 0.f ? m / 448.0f : 1.0f;
+    uint8_t *d = (uint8_t *)dst + r * ldd;
+    for (int c = lane; c < dim; c += 64) d[c] = (uint8_t)f32_to_fp8e4m3_rne(s[c] / scale);
+    if (lane == 0) scales[r] = scale;
+  }
+}
+
+__global__ void fill_f64_kernel(double *p, int n) {
+  const int i = threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// One wave per row: FP64 sums of squared error and of squared reference.
+__global__ __launch_bounds__(256) void error_norms_kernel(const float *__restrict__ C, int64_t ldC,
+                                                          const double *__restrict__ R,
+                                                          int64_t ldR, int64_t rows, int dim,
+                                                          double *__restrict__ out) {
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (r >= rows) return;
+  double se = 0.0, sr = 0.0;
+  for (int c = lane; c < dim; c += 64) {
+    const double ref = R[r * ldR + c];
+    const double d = (double)C[r * ldC + c] - ref;
+    se += d * d;
+    sr += ref * ref;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    se += __shfl_xor(se, off);
+    sr += __shfl_xor(sr, off);
+  }
+  if (lane == 0) {
+    unsafeAtomicAdd(out + 0, se);
+    unsafeAtomicAdd(out + 1, sr);
+    if (sr > 0.0) {
+      // Non-negative doubles order like their bit patterns.
+      const double ratio = se / sr;
+      atomicMax(reinterpret_cast<unsigned long long *>(out + 2),
+                (unsigned long long)__double_as_longlong(ratio));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Dispatch helpers.
+// ---------------------------------------------------------------------------
+struct Shape {
+  int epl, g, nv;
+};
+
+// Lanes per row group and vectors per lane for a dim; vector form needs dim,
+// the leading dimensions and the base pointers to be multiples of 4 elements.
+bool pick_shape(int dim, bool vec_ok, Shape *s) {
+  if (dim < 1) return false;
+  if (vec_ok && dim % 4 == 0) {
+    const int chunks = dim / 4;
+    s->epl = 4;
+    if (chunks <= 8) { s->g = 8; s->nv = 1; }
+    else if (chunks <= 16) { s->g = 16; s->nv = 1; }
+    else if (chunks <= 32) { s->g = 32; s->nv = 1; }
+    else if (chunks <= 64) { s->g = 64; s->nv = 1; }
+    else if (chunks <= 128) { s->g = 64; s->nv = 2; }
+    else if (chunks <= 256) { s->g = 64; s->nv = 4; }
+    else return false;
+    return true;
+  }
+  s->epl = 1;
+  s->g = 64;
+  if (dim <= 64) s->nv = 1;
+  else if (dim <= 128) s->nv = 2;
+  else if (dim <= 256) s->nv = 4;
+  else if (dim <= 512) s->nv = 8;
+  else return false;
+  return true;
+}
+
+bool aligned(const void *p, size_t bytes) { return ((uintptr_t)p % bytes) == 0; }
+
+cola_status hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return COLA_OK;
+  return fail(COLA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int device_cu_count() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cached[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      n = 256;
+    cached[dev] = n;
+  }
+  return cached[dev];
+}
+
+template <typename T>
+cola_status upload(const std::vector<T> &v, T **d) {
+  *d = nullptr;
+  if (v.empty()) return COLA_OK;
+  cola_status st = hip_check(hipMalloc((void **)d, v.size() * sizeof(T)), "hipMalloc");
+  if (st) return st;
+  return hip_check(hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice),
+                   "hipMemcpy H2D");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Handles.
+// ---------------------------------------------------------------------------
+struct cola_zip_s {
+  int32_t *ptr = nullptr, *idx = nullptr, *zip = nullptr, *pack_idx = nullptr,
+          *uniq = nullptr;
+  float *val = nullptr;
+  int64_t nptr = 0, nnz = 0, n_unique = 0;
+};
+
+struct cola_plan_s {
+  int32_t num_rows = 0, nodesPerPE = 0, U_r = 0, U_l = 0, max_dim = 0;
+  int64_t nnz = 0, num_items = 0, num_empty = 0, num_hubs = 0, num_chunks = 0;
+  int4 *items = nullptr;
+  int32_t *xcd_begin = nullptr, *cols = nullptr, *empty_rows = nullptr,
+          *hub_rows = nullptr, *hub_ptr = nullptr, *remote_cols = nullptr;
+  float *vals = nullptr, *partials = nullptr;
+  int64_t device_bytes = 0;
+  int block_threads = 256, blocks_per_cu = 8;
+};
+
+static void free_zip(cola_zip_s *z) {
+  if (!z) return;
+  (void)hipFree(z->ptr);
+  (void)hipFree(z->idx);
+  (void)hipFree(z->zip);
+  (void)hipFree(z->val);
+  (void)hipFree(z->pack_idx);
+  (void)hipFree(z->uniq);
+  delete z;
+}
+
+static void free_plan(cola_plan_s *p) {
+  if (!p) return;
+  (void)hipFree(p->items);
+  (void)hipFree(p->xcd_begin);
+  (void)hipFree(p->cols);
+  (void)hipFree(p->vals);
+  (void)hipFree(p->empty_rows);
+  (void)hipFree(p->hub_rows);
+  (void)hipFree(p->hub_ptr);
+  (void)hipFree(p->remote_cols);
+  (void)hipFree(p->partials);
+  delete p;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+namespace {
+
+template <int EPL, int G, int NV>
+void launch_ppl0(float *out, const float *in, const float *packed, const int32_t *rp,
+                 const int32_t *ci, const int32_t *ri, const float *vl, const int32_t *cp,
+                 const int32_t *rr, const int32_t *pi, const float *vr, int llb, int rlb,
+                 int L, int R, int dim, int peid, hipStream_t s) {
+  const int64_t threads = ((int64_t)L + R) * G;
+  const int64_t blocks = (threads + 255) / 256;
+  hipLaunchKernelGGL((ppl0_kernel<EPL, G, NV>), dim3((unsigned)blocks), dim3(256), 0, s, out,
+                     in, packed, rp, ci, ri, vl, cp, rr, pi, vr, llb, rlb, L, R, dim, peid);
+}
+
+template <int DT, int EPL, int G, int NV>
+void launch_plan_rows(const cola_plan_s *p, float *C, int64_t ldC, const void *Bl, int64_t ldB,
+                      const float *sl, const void *Bp, int64_t ldP, const float *sp, int dim,
+                      int accumulate, int kahan, hipStream_t s) {
+  using T = typename Row<DT, EPL>::T;
+  // Loads in flight per row group: about 16 entries per wave per batch,
+  // fewer when each entry already spans several vectors per lane.
+  constexpr int S = 64 / G;
+  constexpr int U0 = (16 / S) < 1 ? 1 : (16 / S);
+  constexpr int U1 = U0 > 8 ? 8 : U0;
+  constexpr int U = NV > 1 ? ((U1 / NV) < 1 ? 1 : U1 / NV) : U1;
+  const int wpb = p->block_threads / 64;
+  int64_t blocks = (p->num_items + wpb - 1) / wpb;
+  blocks = std::min<int64_t>(blocks, (int64_t)device_cu_count() * p->blocks_per_cu);
+  blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
+  hipLaunchKernelGGL((plan_rows_kernel<DT, EPL, G, NV, U>), dim3((unsigned)blocks),
+                     dim3(p->block_threads), 0, s, p->items, p->xcd_begin, p->cols, p->vals,
+                     (const T *)Bl, ldB, sl, (const T *)Bp, ldP, sp, C, ldC, p->partials,
+                     p->max_dim, dim, accumulate, kahan);
+}
+
+template <int DT>
+bool dispatch_plan(const Shape &sh, const cola_plan_s *p, float *C, int64_t ldC, const void *Bl,
+                   int64_t ldB, const float *sl, const void *Bp, int64_t ldP, const float *sp,
+                   int dim, int accumulate, int kahan, hipStream_t s) {
+#define COLA_PLAN_CASE(E, G_, NV_)                                                          \
+  if (sh.epl == E && sh.g == G_ && sh.nv == NV_) {                                          \
+    launch_plan_rows<DT, E, G_, NV_>(p, C, ldC, Bl, ldB, sl, Bp, ldP, sp, dim, accumulate, \
+                                     kahan, s);                                             \
+    return true;                                                                            \
+  }
+  COLA_PLAN_CASE(4, 8, 1)
+  COLA_PLAN_CASE(4, 16, 1)
+  COLA_PLAN_CASE(4, 32, 1)
+  COLA_PLAN_CASE(4, 64, 1)
+  COLA_PLAN_CASE(4, 64, 2)
+  COLA_PLAN_CASE(4, 64, 4)
+  COLA_PLAN_CASE(1, 64, 1)
+  COLA_PLAN_CASE(1, 64, 2)
+  COLA_PLAN_CASE(1, 64, 4)
+  COLA_PLAN_CASE(1, 64, 8)
+#undef COLA_PLAN_CASE
+  return false;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI (device part).
+// ---------------------------------------------------------------------------
+extern "C" {
+
+cola_status cola_zip_create(const int32_t *ptr, int64_t nptr, const int32_t *idx, int64_t nnz,
+                            const int32_t *zip, const float *val, cola_zip *out) {
+  if (!out) return fail(COLA_ERR_INVALID, "zip_create: out is NULL");
+  *out = nullptr;
+  if (nptr < 0 || nnz < 0 || (nptr > 0 && !ptr) || (nnz > 0 && !idx) || (nptr > 1 && !zip))
+    return fail(COLA_ERR_INVALID, "zip_create: NULL array or negative size");
+  const int64_t nz = nptr > 0 ? nptr - 1 : 0;
+  if (nptr > 0) {
+    if (ptr[0] != 0 || ptr[nptr - 1] != nnz) return fail(COLA_ERR_INVALID, "zip_create: ptr must span [0, nnz]");
+    for (int64_t i = 0; i < nz; ++i)
+      if (ptr[i + 1] < ptr[i]) return fail(COLA_ERR_INVALID, "zip_create: ptr decreasing");
+    for (int64_t i = 1; i < nz; ++i)
+      if (zip[i] < zip[i - 1]) return fail(COLA_ERR_INVALID, "zip_create: zip ids must be sorted");
+  }
+  auto *z = new (std::nothrow) cola_zip_s;
+  if (!z) return fail(COLA_ERR_NOMEM, "zip_create: out of host memory");
+  z->nptr = nptr;
+  z->nnz = nnz;
+  std::vector<int32_t> vptr(ptr, ptr + nptr), vidx(idx, idx + nnz), vzip(zip, zip + nz);
+  std::vector<float> vval(nnz, 1.0f);
+  if (val) std::copy(val, val + nnz, vval.begin());
+  // Packed-row position of every (virtual) entry of zip, and the unique ids.
+  std::vector<int32_t> pack(nz), uniq;
+  for (int64_t i = 0; i < nz; ++i) {
+    if (i == 0 || vzip[i] != vzip[i - 1]) uniq.push_back(vzip[i]);
+    pack[i] = (int32_t)uniq.size() - 1;
+  }
+  z->n_unique = (int64_t)uniq.size();
+  cola_status st;
+  if ((st = upload(vptr, &z->ptr)) || (st = upload(vidx, &z->idx)) || (st = upload(vzip, &z->zip)) ||
+      (st = upload(vval, &z->val)) || (st = upload(pack, &z->pack_idx)) || (st = upload(uniq, &z->uniq))) {
+    free_zip(z);
+    return st;
+  }
+  *out = z;
+  return COLA_OK;
+}
+
+cola_status cola_zip_device_arrays(cola_zip z, int32_t **ptr, int32_t **idx, int32_t **zip,
+                                   float **val, int32_t **pack_idx, int64_t *nptr, int64_t *nnz,
+                                   int64_t *n_unique) {
+  if (!z) return fail(COLA_ERR_INVALID, "zip_device_arrays: NULL handle");
+  if (ptr) *ptr = z->ptr;
+  if (idx) *idx = z->idx;
+  if (zip) *zip = z->zip;
+  if (val) *val = z->val;
+  if (pack_idx) *pack_idx = z->pack_idx;
+  if (nptr) *nptr = z->nptr;
+  if (nnz) *nnz = z->nnz;
+  if (n_unique) *n_unique = z->n_unique;
+  return COLA_OK;
+}
+
+cola_status cola_zip_destroy(cola_zip z) {
+  free_zip(z);
+  return COLA_OK;
+}
+
+cola_status cola_pack_remote(const void *B_global, int64_t ld_B, const float *scales_global,
+                             const int32_t *remote_cols, int64_t num_cols, int dim,
+                             cola_dtype dtype, void *packed, int64_t ld_packed,
+                             float *packed_scales, void *stream) {
+  const size_t eb = dtype_bytes(dtype);
+  if (!eb) return fail(COLA_ERR_UNSUPPORTED, "pack_remote: unknown dtype");
+  if (num_cols < 0 || dim < 1 || ld_B < dim || ld_packed < dim)
+    return fail(COLA_ERR_INVALID, "pack_remote: bad sizes");
+  if (num_cols == 0) return COLA_OK;
+  if (!B_global || !remote_cols || !packed) return fail(COLA_ERR_INVALID, "pack_remote: NULL pointer");
+  if (dtype == COLA_FP8_E4M3 && (!scales_global || !packed_scales))
+    return fail(COLA_ERR_INVALID, "pack_remote: FP8 needs row scales");
+  const int64_t row_bytes = (int64_t)dim * eb;
+  const int64_t ldb = ld_B * eb, ldo = ld_packed * eb;
+  hipStream_t s = (hipStream_t)stream;
+  const bool v16 = row_bytes % 16 == 0 && ldb % 16 == 0 && ldo % 16 == 0 && aligned(B_global, 16) &&
+                   aligned(packed, 16);
+  const float *sc = dtype == COLA_FP8_E4M3 ? scales_global : nullptr;
+  float *osc = dtype == COLA_FP8_E4M3 ? packed_scales : nullptr;
+  const int64_t max_blocks = (int64_t)device_cu_count() * 16;
+  if (v16) {
+    const int64_t vecs = row_bytes / 16;
+    int lpr = 1;
+    while (lpr < vecs && lpr < 64) lpr <<= 1;
+    const int64_t blocks = std::min<int64_t>(max_blocks, (num_cols * lpr + 255) / 256);
+    hipLaunchKernelGGL((pack_rows_kernel<uint4>), dim3((unsigned)blocks), dim3(256), 0, s,
+                       (const char *)B_global, ldb, remote_cols, num_cols, vecs, (char *)packed, ldo,
+                       sc, osc, lpr);
+  } else {
+    const int64_t vecs = row_bytes / (int64_t)eb;
+    int lpr = 1;
+    while (lpr < vecs && lpr < 64) lpr <<= 1;
+    const int64_t blocks = std::min<int64_t>(max_blocks, (num_cols * lpr + 255) / 256);
+    if (eb == 4)
+      hipLaunchKernelGGL((pack_rows_kernel<uint32_t>), dim3((unsigned)blocks), dim3(256), 0, s,
+                         (const char *)B_global, ldb, remote_cols, num_cols, vecs, (char *)packed,
+                         ldo, sc, osc, lpr);
+    else if (eb == 2)
+      hipLaunchKernelGGL((pack_rows_kernel<uint16_t>), dim3((unsigned)blocks), dim3(256), 0, s,
+                         (const char *)B_global, ldb, remote_cols, num_cols, vecs, (char *)packed,
+                         ldo, sc, osc, lpr);
+    else
+      hipLaunchKernelGGL((pack_rows_kernel<uint8_t>), dim3((unsigned)blocks), dim3(256), 0, s,
+                         (const char *)B_global, ldb, remote_cols, num_cols, vecs, (char *)packed,
+                         ldo, sc, osc, lpr);
+  }
+  return hip_check(hipGetLastError(), "pack_remote launch");
+}
+
+cola_status cola_ppl0_batch_spmm(float *output, const float *input, const float *packed,
+                                 const int32_t *csr_rowPtr_l, const int32_t *csr_colIdx_l,
+                                 const int32_t *csr_rowIdx_zip_l, const float *csr_val_l,
+                                 const int32_t *csc_colPtr_r, const int32_t *csc_rowIdx_r,
+                                 const int32_t *csc_packIdx_r, const float *csc_val_r,
+                                 int local_row_lb, int remote_col_lb, int local_row_num,
+                                 int remote_col_num, int dim, int nodesPerPE, int peid,
+                                 void *stream) {
+  if (local_row_num < 0 || remote_col_num < 0 || local_row_lb < 0 || remote_col_lb < 0 ||
+      nodesPerPE < 0)
+    return fail(COLA_ERR_INVALID, "ppl0_batch_spmm: negative range");
+  if (local_row_num + (int64_t)remote_col_num == 0) return COLA_OK;  // cuh:232
+  if (!output) return fail(COLA_ERR_INVALID, "ppl0_batch_spmm: output is NULL");
+  if (local_row_num > 0 && (!input || !csr_rowPtr_l || !csr_colIdx_l || !csr_rowIdx_zip_l || !csr_val_l))
+    return fail(COLA_ERR_INVALID, "ppl0_batch_spmm: NULL local array");
+  if (remote_col_num > 0 && (!packed || !csc_colPtr_r || !csc_rowIdx_r || !csc_packIdx_r || !csc_val_r))
+    return fail(COLA_ERR_INVALID, "ppl0_batch_spmm: NULL remote array");
+  const bool vec_ok = aligned(output, 16) && (!input || aligned(input, 16)) &&
+                      (!packed || aligned(packed, 16));
+  Shape sh;
+  if (!pick_shape(dim, vec_ok, &sh))
+    return fail(COLA_ERR_UNSUPPORTED, "ppl0_batch_spmm: dim " + std::to_string(dim) + " not supported");
+  hipStream_t s = (hipStream_t)stream;
+#define COLA_PPL0_CASE(E, G_, NV_)                                                             \
+  if (sh.epl == E && sh.g == G_ && sh.nv == NV_) {                                             \
+    launch_ppl0<E, G_, NV_>(output, input, packed, csr_rowPtr_l, csr_colIdx_l, csr_rowIdx_zip_l, \
+                            csr_val_l, csc_colPtr_r, csc_rowIdx_r, csc_packIdx_r, csc_val_r,    \
+                            local_row_lb, remote_col_lb, local_row_num, remote_col_num, dim,    \
+                            peid, s);                                                           \
+    return hip_check(hipGetLastError(), "ppl0_batch_spmm launch");                             \
+  }
+  COLA_PPL0_CASE(4, 8, 1)
+  COLA_PPL0_CASE(4, 16, 1)
+  COLA_PPL0_CASE(4, 32, 1)
+  COLA_PPL0_CASE(4, 64, 1)
+  COLA_PPL0_CASE(4, 64, 2)
+  COLA_PPL0_CASE(4, 64, 4)
+  COLA_PPL0_CASE(1, 64, 1)
+  COLA_PPL0_CASE(1, 64, 2)
+  COLA_PPL0_CASE(1, 64, 4)
+  COLA_PPL0_CASE(1, 64, 8)
+#undef COLA_PPL0_CASE
+  return fail(COLA_ERR_UNSUPPORTED, "ppl0_batch_spmm: no kernel for this dim");
+}
+
+cola_status cola_dist_spmm_ppl_batch(float *output, const float *B_global, float *packed,
+                                     cola_zip csr_zip_l, cola_zip csc_zip_r, int dim,
+                                     int nodesPerPE, int peid, void *stream) {
+  if (!csr_zip_l || !csc_zip_r) return fail(COLA_ERR_INVALID, "dist_spmm_ppl_batch: NULL handle");
+  if (!B_global || dim < 1 || nodesPerPE < 0 || peid < 0)
+    return fail(COLA_ERR_INVALID, "dist_spmm_ppl_batch: bad arguments");
+  const int64_t L = csr_zip_l->nptr > 0 ? csr_zip_l->nptr - 1 : 0;
+  const int64_t R = csc_zip_r->nptr > 0 ? csc_zip_r->nptr - 1 : 0;
+  if (L > INT32_MAX || R > INT32_MAX) return fail(COLA_ERR_INVALID, "dist_spmm_ppl_batch: too many rows");
+  cola_status st = cola_pack_remote(B_global, dim, nullptr, csc_zip_r->uniq, csc_zip_r->n_unique, dim,
+                                    COLA_F32, packed, dim, nullptr, stream);
+  if (st) return st;
+  const float *input = B_global + (int64_t)peid * nodesPerPE * dim;
+  return cola_ppl0_batch_spmm(output, input, packed, csr_zip_l->ptr, csr_zip_l->idx, csr_zip_l->zip,
+                              csr_zip_l->val, csc_zip_r->ptr, csc_zip_r->idx, csc_zip_r->pack_idx,
+                              csc_zip_r->val, 0, 0, (int)L, (int)R, dim, nodesPerPE, peid, stream);
+}
+
+cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, cola_plan *out) {
+  if (!part || !out) return fail(COLA_ERR_INVALID, "plan_create: NULL argument");
+  *out = nullptr;
+  cola_plan_opts o{};
+  if (opts) o = *opts;
+  if (o.hub_threshold < 0 || o.chunk_nnz < 0 || o.kahan_degree < 0 || o.max_dim < 0)
+    return fail(COLA_ERR_INVALID, "plan_create: negative option");
+  cola::HostPlan hp;
+  cola_status st = cola::build_plan(part->p, o, &hp);
+  if (st) return st;
+  auto *p = new (std::nothrow) cola_plan_s;
+  if (!p) return fail(COLA_ERR_NOMEM, "plan_create: out of host memory");
+  p->num_rows = hp.num_rows;
+  p->nodesPerPE = hp.nodesPerPE;
+  p->U_r = hp.U_r;
+  p->U_l = hp.U_l;
+  p->max_dim = hp.max_dim;
+  p->nnz = hp.nnz;
+  p->num_items = (int64_t)hp.items.size();
+  p->num_empty = (int64_t)hp.empty_rows.size();
+  p->num_hubs = (int64_t)hp.hub_rows.size();
+  p->num_chunks = hp.hub_chunk_ptr.back();
+  std::vector<int4> items(hp.items.size());
+  for (size_t i = 0; i < items.size(); ++i)
+    items[i] = make_int4(hp.items[i].out, hp.items[i].begin, hp.items[i].end, hp.items[i].flags);
+  if ((st = upload(items, &p->items)) || (st = upload(hp.xcd_begin, &p->xcd_begin)) ||
+      (st = upload(hp.cols, &p->cols)) || (st = upload(hp.vals, &p->vals)) ||
+      (st = upload(hp.empty_rows, &p->empty_rows)) || (st = upload(hp.hub_rows, &p->hub_rows)) ||
+      (st = upload(hp.hub_chunk_ptr, &p->hub_ptr)) || (st = upload(hp.remote_cols, &p->remote_cols))) {
+    free_plan(p);
+    return st;
+  }
+  if (p->num_chunks > 0) {
+    st = hip_check(hipMalloc((void **)&p->partials, (size_t)p->num_chunks * p->max_dim * sizeof(float)),
+                   "hipMalloc partials");
+    if (st) {
+      free_plan(p);
+      return st;
+    }
+  }
+  p->device_bytes = (int64_t)(items.size() * sizeof(int4) + hp.xcd_begin.size() * 4 + hp.cols.size() * 4 +
+                              hp.vals.size() * 4 + hp.empty_rows.size() * 4 + hp.hub_rows.size() * 4 +
+                              hp.hub_chunk_ptr.size() * 4 + hp.remote_cols.size() * 4 +
+                              (size_t)p->num_chunks * p->max_dim * 4);
+  *out = p;
+  return COLA_OK;
+}
+
+cola_status cola_plan_get_info(cola_plan p, cola_plan_info *info) {
+  if (!p || !info) return fail(COLA_ERR_INVALID, "plan_info: NULL argument");
+  info->num_rows = p->num_rows;
+  info->num_items = p->num_items;
+  info->num_hub_rows = p->num_hubs;
+  info->num_hub_chunks = p->num_chunks;
+  info->nnz = p->nnz;
+  info->U_r = p->U_r;
+  info->U_l = p->U_l;
+  info->device_bytes = p->device_bytes;
+  return COLA_OK;
+}
+
+cola_status cola_plan_remote_cols(cola_plan p, const int32_t **d_cols) {
+  if (!p || !d_cols) return fail(COLA_ERR_INVALID, "plan_remote_cols: NULL argument");
+  *d_cols = p->remote_cols;
+  return COLA_OK;
+}
+
+cola_status cola_plan_destroy(cola_plan p) {
+  free_plan(p);
+  return COLA_OK;
+}
+
+cola_status cola_spmm_partition(cola_plan p, float *C, int64_t ld_C, const void *B_local,
+                                int64_t ld_B, const float *scales_local, const void *packed,
+                                int64_t ld_packed, const float *packed_scales, int dim,
+                                cola_dtype dtype, int flags, void *stream) {
+  if (!p) return fail(COLA_ERR_INVALID, "spmm_partition: NULL plan");
+  const size_t eb = dtype_bytes(dtype);
+  if (!eb) return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: unknown dtype");
+  if (flags & ~(COLA_OVERWRITE | COLA_KAHAN)) return fail(COLA_ERR_INVALID, "spmm_partition: unknown flag");
+  if (dim < 1 || ld_C < dim || ld_B < dim || (p->U_r > 0 && ld_packed < dim))
+    return fail(COLA_ERR_INVALID, "spmm_partition: bad dim or leading dimension");
+  if (dim > p->max_dim && p->num_chunks > 0)
+    return fail(COLA_ERR_INVALID, "spmm_partition: dim exceeds the plan's max_dim");
+  if (p->num_rows > 0 && !C) return fail(COLA_ERR_INVALID, "spmm_partition: C is NULL");
+  if (p->nnz > 0 && !B_local && p->U_l > 0) return fail(COLA_ERR_INVALID, "spmm_partition: B_local is NULL");
+  if (p->U_r > 0 && !packed) return fail(COLA_ERR_INVALID, "spmm_partition: packed is NULL");
+  if (dtype == COLA_FP8_E4M3 && ((p->U_l > 0 && !scales_local) || (p->U_r > 0 && !packed_scales)))
+    return fail(COLA_ERR_INVALID, "spmm_partition: FP8 needs row scales");
+  const bool vec_ok = ld_C % 4 == 0 && ld_B % 4 == 0 && (p->U_r == 0 || ld_packed % 4 == 0) &&
+                      aligned(C, 16) && aligned(B_local, 4 * eb) &&
+                      (p->U_r == 0 || aligned(packed, 4 * eb));
+  Shape sh;
+  if (!pick_shape(dim, vec_ok, &sh))
+    return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: dim " + std::to_string(dim) + " not supported");
+  hipStream_t s = (hipStream_t)stream;
+  const int accumulate = (flags & COLA_OVERWRITE) ? 0 : 1;
+  const int kahan = (flags & COLA_KAHAN) ? 1 : 0;
+  if (!accumulate && p->num_empty > 0) {
+    const int64_t blocks = (p->num_empty * 64 + 255) / 256;
+    hipLaunchKernelGGL(zero_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p->empty_rows,
+                       (int)p->num_empty, C, ld_C, dim);
+  }
+  if (p->num_items > 0) {
+    bool ok = false;
+    switch (dtype) {
+      case COLA_F32:
+        ok = dispatch_plan<COLA_F32>(sh, p, C, ld_C, B_local, ld_B, scales_local, packed, ld_packed,
+                                     packed_scales, dim, accumulate, kahan, s);
+        break;
+      case COLA_BF16:
+        ok = dispatch_plan<COLA_BF16>(sh, p, C, ld_C, B_local, ld_B, scales_local, packed, ld_packed,
+                                      packed_scales, dim, accumulate, kahan, s);
+        break;
+      case COLA_F16:
+        ok = dispatch_plan<COLA_F16>(sh, p, C, ld_C, B_local, ld_B, scales_local, packed, ld_packed,
+                                     packed_scales, dim, accumulate, kahan, s);
+        break;
+      case COLA_FP8_E4M3:
+        ok = dispatch_plan<COLA_FP8_E4M3>(sh, p, C, ld_C, B_local, ld_B, scales_local, packed,
+                                          ld_packed, packed_scales, dim, accumulate, kahan, s);
+        break;
+    }
+    if (!ok) return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: no kernel for this dim");
+  }
+  if (p->num_hubs > 0) {
+    const int64_t blocks = (p->num_hubs * 64 + 255) / 256;
+    hipLaunchKernelGGL(hub_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p->hub_rows,
+                       p->hub_ptr, (int)p->num_hubs, p->partials, p->max_dim, C, ld_C, dim,
+                       accumulate, kahan);
+  }
+  return hip_check(hipGetLastError(), "spmm_partition launch");
+}
+
+cola_status cola_quantize_rows(const float *src, int64_t ld_src, int64_t rows, int dim,
+                               cola_dtype dtype, void *dst, int64_t ld_dst, float *scales,
+                               void *stream) {
+  if (!dtype_bytes(dtype)) return fail(COLA_ERR_UNSUPPORTED, "quantize_rows: unknown dtype");
+  if (rows < 0 || dim < 1 || ld_src < dim || ld_dst < dim)
+    return fail(COLA_ERR_INVALID, "quantize_rows: bad sizes");
+  if (rows == 0) return COLA_OK;
+  if (!src || !dst || (dtype == COLA_FP8_E4M3 && !scales))
+    return fail(COLA_ERR_INVALID, "quantize_rows: NULL pointer");
+  const int64_t blocks = (rows * 64 + 255) / 256;
+  if (blocks > INT32_MAX) return fail(COLA_ERR_INVALID, "quantize_rows: too many rows");
+  hipLaunchKernelGGL(quantize_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     src, ld_src, rows, dim, (int)dtype, dst, ld_dst, scales);
+  return hip_check(hipGetLastError(), "quantize_rows launch");
+}
+
+cola_status cola_error_norms(const float *C, int64_t ld_C, const double *R, int64_t ld_R,
+                             int64_t rows, int dim, double *out, void *stream) {
+  if (!out || rows < 0 || dim < 1 || ld_C < dim || ld_R < dim)
+    return fail(COLA_ERR_INVALID, "error_norms: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(fill_f64_kernel, dim3(1), dim3(64), 0, s, out, 3);
+  if (rows > 0) {
+    if (!C || !R) return fail(COLA_ERR_INVALID, "error_norms: NULL matrix");
+    const int64_t blocks = (rows * 64 + 255) / 256;
+    hipLaunchKernelGGL(error_norms_kernel, dim3((unsigned)blocks), dim3(256), 0, s, C, ld_C, R, ld_R,
+                       rows, dim, out);
+  }
+  return hip_check(hipGetLastError(), "error_norms launch");
+}
+
+}  // extern "C"

@@ -1,0 +1,205 @@
+"""Device-side host API: device sparse structures, the reference-semantics
+launcher, the degree-binned plan path, packing, precision conversion and
+on-device error norms.  Device buffers are torch tensors on the GPU (torch is
+used only for device memory and streams); every launch goes to the current
+torch stream unless a stream is given.
+
+Reference interfaces mirrored (paths relative to the CoLa tree):
+  include/sparse_data.h:28-77            d_csc_zip / d_csr_zip
+  include/dist_spmm_ppl_batch.cuh:202-245  ppl0_batch_SpMM
+  include/dist_spmm_ppl_batch.cuh:851-918  distSpMM_ppl_batch_wrapper (single stage)
+  include/dist_spmm_ppl_batch.cuh:58-73    batching_data_ppl (-> pack_remote)
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import ACCUMULATE, BF16, F16, F32, FP8_E4M3, KAHAN, OVERWRITE, check, lib  # noqa: F401
+
+DTYPE_BYTES = {F32: 4, BF16: 2, F16: 2, FP8_E4M3: 1}
+DTYPE_NAMES = {F32: "f32", BF16: "bf16", F16: "f16", FP8_E4M3: "fp8_e4m3"}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _dptr(t):
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return ctypes.c_void_p(t)
+    if not t.is_cuda:
+        raise ValueError("expected a GPU tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def _host(a, dtype):
+    a = np.ascontiguousarray(np.asarray(a), dtype=dtype)
+    return a
+
+
+class _Zip:
+    """Device copy of a zip CSR/CSC (hipMalloc + synchronous H2D, freed on delete)."""
+
+    def __init__(self, ptr, idx, zip_ids, val=None):
+        ptr, idx, zid = _host(ptr, np.int32), _host(idx, np.int32), _host(zip_ids, np.int32)
+        va = None if val is None else _host(val, np.float32)
+        h = ctypes.c_void_p()
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None and a.size else None  # noqa: E731
+        check(lib().cola_zip_create(vp(ptr), len(ptr), vp(idx), len(idx), vp(zid), vp(va), ctypes.byref(h)))
+        self._h = h
+        self._read()
+
+    def _read(self):
+        p = [ctypes.c_void_p() for _ in range(5)]
+        n = [ctypes.c_int64() for _ in range(3)]
+        check(lib().cola_zip_device_arrays(self._h, *[ctypes.byref(x) for x in p], *[ctypes.byref(x) for x in n]))
+        self._ptr, self._idx, self._zip, self._val, self._pack = (x.value for x in p)
+        self.nptr, self.nnz, self.n_unique = (x.value for x in n)
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def num(self):
+        """Number of (virtual) rows / columns."""
+        return max(self.nptr - 1, 0)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.cola_zip_destroy(self._h)
+            self._h = None
+
+
+class d_csr_zip(_Zip):
+    """sparse_data.h:53-77: d_csr_rowPtr, d_csr_colIdx, d_csr_rowIdx_zip, d_csr_val."""
+
+    def __init__(self, rowPtr, colIdx, rowIdx_zip, val=None):
+        super().__init__(rowPtr, colIdx, rowIdx_zip, val)
+
+    d_csr_rowPtr = property(lambda s: s._ptr)
+    d_csr_colIdx = property(lambda s: s._idx)
+    d_csr_rowIdx_zip = property(lambda s: s._zip)
+    d_csr_val = property(lambda s: s._val)
+
+
+class d_csc_zip(_Zip):
+    """sparse_data.h:28-52: d_csc_colPtr, d_csc_rowIdx, d_csc_colIdx_zip, d_csc_val,
+    plus d_csc_packIdx (packed-buffer row of each virtual column)."""
+
+    def __init__(self, colPtr, rowIdx, colIdx_zip, val=None):
+        super().__init__(colPtr, rowIdx, colIdx_zip, val)
+
+    d_csc_colPtr = property(lambda s: s._ptr)
+    d_csc_rowIdx = property(lambda s: s._idx)
+    d_csc_colIdx_zip = property(lambda s: s._zip)
+    d_csc_val = property(lambda s: s._val)
+    d_csc_packIdx = property(lambda s: s._pack)
+
+
+def pack_remote(B_global, remote_cols, packed, dim=None, dtype=F32, scales_global=None,
+                packed_scales=None, num_cols=None, stream=None):
+    """packed[u, :dim] = B_global[remote_cols[u], :dim] (row strides from the tensors)."""
+    dim = dim if dim is not None else B_global.shape[1]
+    n = num_cols if num_cols is not None else (remote_cols.numel() if hasattr(remote_cols, "numel") else 0)
+    cols_ptr = remote_cols if isinstance(remote_cols, int) else _dptr(remote_cols)
+    check(lib().cola_pack_remote(_dptr(B_global), B_global.stride(0), _dptr(scales_global), cols_ptr, int(n),
+                                 int(dim), int(dtype), _dptr(packed), packed.stride(0), _dptr(packed_scales),
+                                 _stream(stream)))
+
+
+def ppl0_batch_SpMM(output, input, packed, csr_zip_l, csc_zip_r, local_row_lb, remote_col_lb,
+                    local_row_num, remote_col_num, dim, nodesPerPE, peid, stream=None):
+    """Reference-semantics partition SpMM (FP32 atomics into output)."""
+    check(lib().cola_ppl0_batch_spmm(
+        _dptr(output), _dptr(input), _dptr(packed),
+        csr_zip_l._ptr, csr_zip_l._idx, csr_zip_l._zip, csr_zip_l._val,
+        csc_zip_r._ptr, csc_zip_r._idx, csc_zip_r._pack, csc_zip_r._val,
+        int(local_row_lb), int(remote_col_lb), int(local_row_num), int(remote_col_num),
+        int(dim), int(nodesPerPE), int(peid), _stream(stream)))
+
+
+def dist_spmm_ppl_batch(output, B_global, packed, csr_zip_l, csc_zip_r, dim, nodesPerPE, peid, stream=None):
+    """Single-stage distSpMM_ppl_batch_wrapper: pack the unique remote rows, then
+    accumulate every virtual local row and remote column into output."""
+    check(lib().cola_dist_spmm_ppl_batch(_dptr(output), _dptr(B_global), _dptr(packed), csr_zip_l.handle,
+                                         csc_zip_r.handle, int(dim), int(nodesPerPE), int(peid),
+                                         _stream(stream)))
+
+
+class Plan:
+    """Degree-binned, atomic-free plan of one partition (device resident)."""
+
+    def __init__(self, partition, hub_threshold=0, chunk_nnz=0, kahan_degree=0, max_dim=0):
+        o = _lib.PlanOpts(int(hub_threshold), int(chunk_nnz), int(kahan_degree), int(max_dim))
+        h = ctypes.c_void_p()
+        check(lib().cola_plan_create(partition.handle, ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        i = _lib.PlanInfo()
+        check(lib().cola_plan_get_info(h, ctypes.byref(i)))
+        self.info = {f: getattr(i, f) for f, _ in _lib.PlanInfo._fields_}
+        rc = ctypes.c_void_p()
+        check(lib().cola_plan_remote_cols(h, ctypes.byref(rc)))
+        self.remote_cols_ptr = rc.value or 0
+        self.U_r = self.info["U_r"]
+        self.U_l = self.info["U_l"]
+        self.nnz = self.info["nnz"]
+        self.num_rows = self.info["num_rows"]
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.cola_plan_destroy(self._h)
+            self._h = None
+
+    def pack(self, B_global, packed, dim=None, dtype=F32, scales_global=None, packed_scales=None, stream=None):
+        pack_remote(B_global, self.remote_cols_ptr, packed, dim=dim, dtype=dtype, scales_global=scales_global,
+                    packed_scales=packed_scales, num_cols=self.U_r, stream=stream)
+
+    def spmm(self, C, B_local, packed, dim=None, dtype=F32, flags=ACCUMULATE, scales_local=None,
+             packed_scales=None, stream=None):
+        dim = dim if dim is not None else C.shape[1]
+        ldp = packed.stride(0) if packed is not None and packed.dim() == 2 else dim
+        check(lib().cola_spmm_partition(self._h, _dptr(C), C.stride(0), _dptr(B_local), B_local.stride(0),
+                                        _dptr(scales_local), _dptr(packed), ldp, _dptr(packed_scales),
+                                        int(dim), int(dtype), int(flags), _stream(stream)))
+
+
+def quantize_rows(src, dtype, dst=None, scales=None, stream=None):
+    """FP32 rows -> storage dtype (RNE; FP8 with per-row scale max|x|/448)."""
+    torch = _torch()
+    rows, dim = src.shape
+    tdt = {F32: torch.float32, BF16: torch.uint16, F16: torch.uint16, FP8_E4M3: torch.uint8}[dtype]
+    if dst is None:
+        dst = torch.empty((rows, dim), dtype=tdt, device=src.device)
+    if dtype == FP8_E4M3 and scales is None:
+        scales = torch.empty(rows, dtype=torch.float32, device=src.device)
+    check(lib().cola_quantize_rows(_dptr(src), src.stride(0), rows, dim, int(dtype), _dptr(dst), dst.stride(0),
+                                   _dptr(scales), _stream(stream)))
+    return dst, scales
+
+
+def error_norms(C, R, out=None, stream=None):
+    """(sum (C-R)^2, sum R^2, max_i |C_i-R_i|^2/|R_i|^2) computed on the device in FP64."""
+    torch = _torch()
+    rows, dim = C.shape
+    if out is None:
+        out = torch.empty(3, dtype=torch.float64, device=C.device)
+    check(lib().cola_error_norms(_dptr(C), C.stride(0), _dptr(R), R.stride(0), rows, dim, _dptr(out),
+                                 _stream(stream)))
+    return out

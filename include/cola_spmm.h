@@ -1,0 +1,283 @@
+/*
+ * cola_spmm.h — C ABI of the MI355X (gfx950) single-partition SpMM engine.
+ *
+ * One row partition p of a row-partitioned C = A·B: the partition owns rows
+ * [p·P, min(n,(p+1)·P)) of A (P = nodesPerPE = ceil(n/nGPU)) and the same rows
+ * of B.  A's row panel is stored as a local CSR ("zip": empty rows collapsed,
+ * columns in [0,P) relative to the partition) plus a remote CSC over the
+ * sorted unique remote global columns.  The device computes
+ *
+ *     C[0:M, :] (+)= A_local · B_p  +  A_remote · B
+ *
+ * Conventions (all entry points):
+ *   - every function returns a cola_status (0 = COLA_OK); the text of the
+ *     last failure on the calling thread is returned by cola_last_error().
+ *     Nothing here calls exit().
+ *   - device pointers are caller-owned unless a handle owns them; handles own
+ *     what their create call allocated and free it in destroy.
+ *   - hipStream_t is passed as a plain pointer (void*, NULL = default stream);
+ *     every launch is stream-ordered and asynchronous; the caller synchronises.
+ *   - dense matrices are row-major with a leading dimension (row stride) in
+ *     elements; rows are addressed with 64-bit offsets.
+ *   - a handle may be used from several threads for reads; concurrent
+ *     launches that share a plan's hub workspace on different streams race.
+ *
+ * Reference interfaces each group replaces are cited per declaration
+ * (paths relative to the CoLa source tree).
+ */
+#ifndef COLA_SPMM_H
+#define COLA_SPMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int cola_status;
+enum {
+  COLA_OK = 0,
+  COLA_ERR_INVALID = 1,     /* bad argument, shape or inconsistent arrays   */
+  COLA_ERR_IO = 2,          /* missing / truncated / mis-sized file          */
+  COLA_ERR_HIP = 3,         /* HIP runtime error (launch, alloc, copy)       */
+  COLA_ERR_UNSUPPORTED = 4, /* dim / dtype combination not provided          */
+  COLA_ERR_NOMEM = 5        /* host allocation failed or budget exhausted    */
+};
+
+/* Storage type of the dense B rows (accumulation is always FP32). */
+typedef enum {
+  COLA_F32 = 0,
+  COLA_BF16 = 1,
+  COLA_F16 = 2,
+  COLA_FP8_E4M3 = 3 /* OCP e4m3fn bytes + one FP32 scale per row */
+} cola_dtype;
+
+/* Flags of cola_spmm_partition / cola_dist_spmm. */
+enum {
+  COLA_ACCUMULATE = 0,   /* C += A_p·B   (reference semantics, cuh:141,196)  */
+  COLA_OVERWRITE = 1,    /* C  = A_p·B   (rows without nnz are zeroed)       */
+  COLA_KAHAN = 2         /* compensated FP32 sums on hub rows                */
+};
+
+const char *cola_last_error(void);
+const char *cola_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Host-side graph / partition data (no GPU needed).                         */
+/* ------------------------------------------------------------------------ */
+
+/* Global CSR graph held in host memory (int32 indices, FP32 values). */
+typedef struct cola_graph_s *cola_graph;
+
+/* Seeded R-MAT graph: n = 2^log_n nodes, round(n·avg_degree/2) drawn edges
+ * with quadrant probabilities (a, b, c, 1-a-b-c), each edge added in both
+ * directions, self-loops and duplicates dropped, values 1/sqrt(d_i·d_j)
+ * (GCN normalisation) when gcn_values != 0, else 1.0.
+ * reorder: 0 = natural R-MAT ids, 1 = Cuthill-McKee locality order.
+ * Deterministic for a given seed whatever the host thread count. */
+cola_status cola_rmat_graph(int log_n, double avg_degree, double a, double b,
+                            double c, uint64_t seed, int reorder,
+                            int gcn_values, cola_graph *out);
+
+/* Global CSR from caller arrays (copied).  Columns within a row may come in
+ * any order and may repeat: duplicates are summed, as scipy's tocsr() does
+ * (prep_torch_par.py:93-101). */
+cola_status cola_graph_from_csr(int64_t n, const int64_t *row_ptr,
+                                const int32_t *col_idx, const float *val,
+                                cola_graph *out);
+cola_status cola_graph_info(cola_graph g, int64_t *n, int64_t *nnz,
+                            int64_t *max_degree);
+/* Borrowed pointers, valid until cola_graph_destroy. */
+cola_status cola_graph_arrays(cola_graph g, const int64_t **row_ptr,
+                              const int32_t **col_idx, const float **val);
+cola_status cola_graph_destroy(cola_graph g);
+
+/* One rank's partition in the prep_torch_par.py:24-79 layout, host memory.
+ * Array lengths: csr_rowPtr_l L+1, csr_colIdx_l/csr_val_l nnz_l,
+ * csr_rowIdx_l L, csc_colPtr_r U_r+1, csc_rowIdx_r/csc_val_r nnz_r,
+ * csc_colIdx_r U_r. */
+typedef struct cola_partition_s *cola_partition;
+typedef struct {
+  int64_t num_nodes;   /* n                                      */
+  int32_t nGPU;        /* number of row partitions               */
+  int32_t rank;        /* this partition                         */
+  int32_t nodesPerPE;  /* P = ceil(n / nGPU)                     */
+  int32_t num_rows;    /* M = min(n, (rank+1)·P) − rank·P        */
+  int32_t L;           /* non-empty local rows                   */
+  int32_t U_r;         /* unique remote columns                  */
+  int64_t nnz_l;       /* local nnz                              */
+  int64_t nnz_r;       /* remote nnz                             */
+  int32_t U_l;         /* distinct local columns referenced      */
+  int32_t max_row_nnz; /* max (local+remote) nnz of one row      */
+} cola_partition_info;
+
+typedef struct {
+  const int32_t *csr_rowPtr_l, *csr_colIdx_l, *csr_rowIdx_l;
+  const float *csr_val_l;
+  const int32_t *csc_colPtr_r, *csc_rowIdx_r, *csc_colIdx_r;
+  const float *csc_val_r;
+} cola_partition_arrays;
+
+/* prep_torch_par.py:24-79 row_partition for one rank, in memory. */
+cola_status cola_row_partition(cola_graph g, int nGPU, int rank,
+                               cola_partition *out);
+/* Partition from caller arrays (copied, validated like a load). */
+cola_status cola_partition_from_arrays(int64_t num_nodes, int nGPU, int rank,
+                                       const cola_partition_arrays *arrs,
+                                       int64_t L, int64_t nnz_l, int64_t U_r,
+                                       int64_t nnz_r, cola_partition *out);
+cola_status cola_partition_get_info(cola_partition p, cola_partition_info *info);
+cola_status cola_partition_get_arrays(cola_partition p,
+                                      cola_partition_arrays *arrs);
+cola_status cola_partition_destroy(cola_partition p);
+
+/* prep_torch_par.py:93-101 serialize: writes <bin_path>/<graph>/global_rowPtr.bin,
+ * global_colIdx.bin and, for every rank, GPU<nGPU>/{csr_rowPtr_l,csr_colIdx_l,
+ * csr_rowIdx_l,csc_colPtr_r,csc_rowIdx_r,csc_colIdx_r}_<nGPU>_<rank>.bin
+ * (raw little-endian int32).  write_values != 0 adds csr_val_l_* / csc_val_r_*
+ * (FP32) value files, which the reference does not have. */
+cola_status cola_serialize(cola_graph g, const char *bin_path,
+                           const char *graph_name, int nGPU, int write_values);
+
+/* distSpMM_ppl_batch.cu:49-99 partition load: reads global_rowPtr.bin (for n)
+ * and the six per-rank files with size and range checks (bin2vec,
+ * dist_spmm_ppl_batch.cuh:7-27, without its 2 GiB int limit); value files are
+ * used when present, else every value is 1.0 (distSpMM_ppl_batch.cu:83,99). */
+cola_status cola_partition_load(const char *bin_path, const char *graph_name,
+                                int nGPU, int rank, cola_partition *out);
+
+/* dist_spmm_ppl_batch.cuh:29-56 nnz_split.  rowIdx (nrows) and rowPtr
+ * (nrows+1) describe a zip CSR/CSC with nnz entries; rows are cut into
+ * virtual rows of gran = ceil((nnz/nrows)/32)·32 entries.  Two-call protocol:
+ * pass target=NULL/ptr=NULL to get *n_virtual, then arrays of n_virtual and
+ * n_virtual+1 entries. */
+cola_status cola_nnz_split(const int32_t *rowIdx, int64_t nrows,
+                           const int32_t *rowPtr, int64_t nnz,
+                           int32_t *target, int32_t *ptr, int64_t *n_virtual,
+                           int32_t *gran);
+
+/* ------------------------------------------------------------------------ */
+/* Device data (sparse_data.h:6-77 d_csr / d_csr_zip / d_csc_zip).           */
+/* ------------------------------------------------------------------------ */
+
+/* Device copy of a zip CSR (d_csr_zip, sparse_data.h:53-77) or zip CSC
+ * (d_csc_zip, sparse_data.h:28-52): hipMalloc + synchronous H2D of the four
+ * arrays.  For the CSC, colIdx_zip holds global remote column ids and the
+ * handle also stores, per (virtual) column, the row of the packed remote
+ * buffer it reads (the position of its column among the unique ids). */
+typedef struct cola_zip_s *cola_zip;
+cola_status cola_zip_create(const int32_t *ptr, int64_t nptr,
+                            const int32_t *idx, int64_t nnz,
+                            const int32_t *zip, const float *val,
+                            cola_zip *out);
+/* Device pointers (borrowed): ptr, idx, zip, val and, for a CSC, pack_idx. */
+cola_status cola_zip_device_arrays(cola_zip z, int32_t **ptr, int32_t **idx,
+                                   int32_t **zip, float **val,
+                                   int32_t **pack_idx, int64_t *nptr,
+                                   int64_t *nnz, int64_t *n_unique);
+cola_status cola_zip_destroy(cola_zip z);
+
+/* ------------------------------------------------------------------------ */
+/* Hot path.                                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* Gather-pack of the unique remote B rows (replaces the per-warp NVSHMEM get,
+ * dist_spmm_ppl_batch.cuh:120-125, and the request-coalescing pack kernel
+ * batching_data_ppl, dist_spmm_ppl_batch.cuh:58-73):
+ *   packed[u, 0:dim] = B_global[remote_cols[u], 0:dim],  u in [0, num_cols)
+ * B_global holds every partition's slice back to back (row g of the graph is
+ * row g of B_global).  For COLA_FP8_E4M3 the per-row scales travel too
+ * (scales_global[g] -> packed_scales[u]); pass NULL for other dtypes. */
+cola_status cola_pack_remote(const void *B_global, int64_t ld_B,
+                             const float *scales_global,
+                             const int32_t *remote_cols, int64_t num_cols,
+                             int dim, cola_dtype dtype, void *packed,
+                             int64_t ld_packed, float *packed_scales,
+                             void *stream);
+
+/* Reference-semantics partition SpMM (replaces ppl0_batch_SpMM<CF,res>,
+ * dist_spmm_ppl_batch.cuh:202-245, and its kernel cuh:76-200): virtual local
+ * rows [local_row_lb, local_row_lb+local_row_num) of a zip CSR and virtual
+ * remote columns [remote_col_lb, remote_col_lb+remote_col_num) of a zip CSC
+ * are accumulated into `output` (P × dim, row-major, ld = dim) with FP32
+ * atomics.  Local columns index `input` (this partition's B slice, ld = dim);
+ * remote column v reads packed row csc_packIdx_r[v] (filled by
+ * cola_pack_remote over the unique remote columns).  Any dim >= 1 is
+ * computed (the reference computes only dim <= 64 and 97..128). */
+cola_status cola_ppl0_batch_spmm(
+    float *output, const float *input, const float *packed,
+    const int32_t *csr_rowPtr_l, const int32_t *csr_colIdx_l,
+    const int32_t *csr_rowIdx_zip_l, const float *csr_val_l,
+    const int32_t *csc_colPtr_r, const int32_t *csc_rowIdx_r,
+    const int32_t *csc_packIdx_r, const float *csc_val_r, int local_row_lb,
+    int remote_col_lb, int local_row_num, int remote_col_num, int dim,
+    int nodesPerPE, int peid, void *stream);
+
+/* Single-stage distSpMM_ppl_batch_wrapper (dist_spmm_ppl_batch.cuh:851-918
+ * with group_size == nranks <= 8, cuh:837-848): pack the unique remote rows
+ * of `csc_zip_r` from B_global into `packed`, then cola_ppl0_batch_spmm over
+ * every virtual row / column.  input = B_global + peid·nodesPerPE·dim. */
+cola_status cola_dist_spmm_ppl_batch(float *output, const float *B_global,
+                                     float *packed, cola_zip csr_zip_l,
+                                     cola_zip csc_zip_r, int dim,
+                                     int nodesPerPE, int peid, void *stream);
+
+/* Degree-binned, atomic-free partition plan (replaces nnz_split's virtual
+ * rows for the fast path): for every owned row, local nnz and remote nnz are
+ * merged into one row list whose entries address either B_p (local column)
+ * or the packed remote buffer (unique-column position).  Rows above
+ * hub_threshold nnz are cut into chunks of chunk_nnz entries whose partial
+ * sums are reduced in a fixed order by a second kernel, so results do not
+ * depend on scheduling.  Work items are laid out so that each of the 8 XCDs
+ * walks a contiguous row range. */
+typedef struct cola_plan_s *cola_plan;
+typedef struct {
+  int32_t hub_threshold; /* rows with more nnz are split (0 = default 1024)  */
+  int32_t chunk_nnz;     /* nnz per hub chunk (0 = default 512)              */
+  int32_t kahan_degree;  /* rows above this degree use compensated sums when
+                            COLA_KAHAN is passed (0 = default 1024)         */
+  int32_t max_dim;       /* largest dim the hub workspace is sized for
+                            (0 = default 256)                               */
+} cola_plan_opts;
+cola_status cola_plan_create(cola_partition p, const cola_plan_opts *opts,
+                             cola_plan *out);
+typedef struct {
+  int64_t num_rows, num_items, num_hub_rows, num_hub_chunks, nnz, U_r, U_l;
+  int64_t device_bytes;
+} cola_plan_info;
+cola_status cola_plan_get_info(cola_plan plan, cola_plan_info *info);
+/* Device array of the unique remote column ids (U_r entries), for packing. */
+cola_status cola_plan_remote_cols(cola_plan plan, const int32_t **d_cols);
+cola_status cola_plan_destroy(cola_plan plan);
+
+/* C[0:M,:] (+)= A_p·B through a plan.  B_local: this partition's slice
+ * (P rows, ld_B); packed: the U_r packed remote rows (ld_packed).  For
+ * COLA_FP8_E4M3, scales_local / packed_scales are the per-row FP32 scales.
+ * C is FP32, ld_C >= dim. */
+cola_status cola_spmm_partition(cola_plan plan, float *C, int64_t ld_C,
+                                const void *B_local, int64_t ld_B,
+                                const float *scales_local, const void *packed,
+                                int64_t ld_packed, const float *packed_scales,
+                                int dim, cola_dtype dtype, int flags,
+                                void *stream);
+
+/* Row-wise conversion of FP32 rows to a storage dtype (round to nearest
+ * even; FP8 rows get scale = max|row|/448 and are stored as x/scale).
+ * dst has ld_dst elements per row; scales (FP8 only) has rows entries. */
+cola_status cola_quantize_rows(const float *src, int64_t ld_src, int64_t rows,
+                               int dim, cola_dtype dtype, void *dst,
+                               int64_t ld_dst, float *scales, void *stream);
+
+/* On-device FP64 error norms of C (FP32, ld_C) against a reference R (FP64,
+ * ld_R) over rows × dim: out[0] = Σ(C−R)², out[1] = ΣR², out[2] =
+ * max over rows of ‖C_i−R_i‖²/‖R_i‖² (rows with R_i = 0 skipped).
+ * out is 3 doubles of device memory. */
+cola_status cola_error_norms(const float *C, int64_t ld_C, const double *R,
+                             int64_t ld_R, int64_t rows, int dim, double *out,
+                             void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COLA_SPMM_H */

@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every entry point include/*.h declares
+(no compute calls: runs without a GPU)."""
+import ctypes
+import glob
+import os
+import re
+
+import cola_amd
+from cola_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", "", text)
+        names |= set(re.findall(r"\b(cola_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    L = cola_amd.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_symbol():
+    assert declared_functions() == set(_lib.SIGNATURES)
+
+
+def test_exports_are_c_symbols():
+    out = os.popen(f"nm -D --defined-only {_lib.LIB_PATH}").read()
+    exported = set(re.findall(r" T (cola_[a-z0-9_]+)$", out, flags=re.M))
+    assert declared_functions() <= exported
+
+
+def test_version_and_error_text():
+    L = cola_amd.lib()
+    assert L.cola_version().startswith(b"cola")
+    # An invalid call returns a status and sets the error text (no exit()).
+    h = ctypes.c_void_p()
+    st = L.cola_rmat_graph(0, 1.0, 0.5, 0.2, 0.2, 1, 0, 1, ctypes.byref(h))
+    assert st == _lib.COLA_ERR_INVALID
+    assert b"log_n" in L.cola_last_error()

@@ -1,0 +1,361 @@
+"""GPU parity tests: the HIP paths through the C ABI against the CPU oracle.
+
+Bars: FP32 storage -> relative Frobenius error <= 1e-5 against the FP64 oracle
+(the stated tolerance for the plain path); integer / byte work (packing,
+conversions) -> bit-exact; reduced-precision storage -> the error bounds
+stated per dtype below.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import cola_amd as cola  # noqa: E402
+from oracle import oracle as orc  # noqa: E402
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a GPU")]
+
+FP32_TOL = 1e-5
+
+
+def rel_fro(C, R):
+    d = np.linalg.norm(np.asarray(C, np.float64) - R)
+    r = np.linalg.norm(R)
+    return d / r if r > 0 else d
+
+
+class Case:
+    """A graph, one partition of it, a padded B and the FP64 oracle product."""
+
+    def __init__(self, graph, nGPU, rank, dim, seed=0, b_ones=False):
+        self.g = graph
+        self.n = graph.n
+        self.rp, self.ci, self.va = graph.arrays()
+        self.part = graph.partition(nGPU, rank)
+        self.info = self.part.info()
+        self.P, self.M = self.info["nodesPerPE"], self.info["num_rows"]
+        self.nGPU, self.rank, self.dim = nGPU, rank, dim
+        rng = np.random.default_rng(seed)
+        rows = self.P * nGPU
+        self.Bh = np.ones((rows, dim), np.float32) if b_ones else rng.uniform(-1, 1, (rows, dim)).astype(np.float32)
+        lb = rank * self.P
+        self.ref = orc.spmm_f64(self.rp, self.ci, self.va, lb, lb + self.M, self.Bh[:self.n])
+        self.dev = torch.device("cuda:0")
+        self.B = torch.from_numpy(self.Bh).to(self.dev)
+
+    def B_local(self):
+        return self.B[self.rank * self.P:(self.rank + 1) * self.P]
+
+    def run_plan(self, flags=cola.OVERWRITE, init=None, **opts):
+        plan = cola.Plan(self.part, **opts)
+        packed = torch.empty((max(plan.U_r, 1), self.dim), dtype=torch.float32, device=self.dev)
+        C = (torch.zeros if init is None else torch.full)((self.P, self.dim), *(() if init is None else (init,)),
+                                                          dtype=torch.float32, device=self.dev)
+        plan.pack(self.B, packed)
+        plan.spmm(C, self.B_local(), packed, flags=flags)
+        torch.cuda.synchronize()
+        return C.cpu().numpy(), plan
+
+    def run_reference_path(self, split=True):
+        a = self.part.arrays()
+        if split:
+            tl, pl, _ = cola.nnz_split(a["csr_rowIdx_l"], a["csr_rowPtr_l"], len(a["csr_colIdx_l"]))
+            tr, pr, _ = cola.nnz_split(a["csc_colIdx_r"], a["csc_colPtr_r"], len(a["csc_rowIdx_r"]))
+        else:
+            tl, pl = a["csr_rowIdx_l"], a["csr_rowPtr_l"]
+            tr, pr = a["csc_colIdx_r"], a["csc_colPtr_r"]
+        zl = cola.d_csr_zip(pl, a["csr_colIdx_l"], tl, a["csr_val_l"])
+        zr = cola.d_csc_zip(pr, a["csc_rowIdx_r"], tr, a["csc_val_r"])
+        packed = torch.empty((max(zr.n_unique, 1), self.dim), dtype=torch.float32, device=self.dev)
+        C = torch.zeros((self.P, self.dim), dtype=torch.float32, device=self.dev)
+        cola.dist_spmm_ppl_batch(C, self.B, packed, zl, zr, self.dim, self.P, self.rank)
+        torch.cuda.synchronize()
+        return C.cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def c1():
+    """The CPU-runnable configuration: R-MAT 2^16, degree 16, GCN values,
+    partition 0 of 4, dim 64."""
+    return Case(cola.Graph.rmat(16, 16, seed=1), 4, 0, 64, seed=11)
+
+
+def test_plan_path_c1(c1):
+    C, plan = c1.run_plan()
+    err = rel_fro(C[:c1.M], c1.ref)
+    print(f"plan path C1 rel_fro={err:.3e} items={plan.info['num_items']} hubs={plan.info['num_hub_rows']}")
+    assert err <= FP32_TOL
+    assert np.all(C[c1.M:] == 0)
+
+
+def test_reference_path_c1(c1):
+    C = c1.run_reference_path(split=True)
+    err = rel_fro(C[:c1.M], c1.ref)
+    print(f"reference path C1 rel_fro={err:.3e}")
+    assert err <= FP32_TOL
+    C2 = c1.run_reference_path(split=False)
+    assert rel_fro(C2[:c1.M], c1.ref) <= FP32_TOL
+
+
+def test_reference_path_matches_fp32_restatement(c1):
+    """The HIP reference path against the oracle's FP32 restatement of
+    ppl0_batch_SpMM_kernel on the same virtual rows/columns."""
+    a = c1.part.arrays()
+    tl, pl, _ = cola.nnz_split(a["csr_rowIdx_l"], a["csr_rowPtr_l"], len(a["csr_colIdx_l"]))
+    tr, pr, _ = cola.nnz_split(a["csc_colIdx_r"], a["csc_colPtr_r"], len(a["csc_rowIdx_r"]))
+    Cr = np.zeros((c1.P, c1.dim), np.float32)
+    orc.ppl0_f32(Cr, c1.Bh, c1.dim, c1.P, c1.rank, pl, a["csr_colIdx_l"], tl, a["csr_val_l"], pr,
+                 a["csc_rowIdx_r"], tr, a["csc_val_r"])
+    C = c1.run_reference_path(split=True)
+    assert rel_fro(C, Cr.astype(np.float64)) <= FP32_TOL
+
+
+def test_accumulate_semantics(c1):
+    """C is accumulated into (cuh:141,196): C0 + A_p·B."""
+    C, _ = c1.run_plan(flags=cola.ACCUMULATE, init=2.5)
+    assert rel_fro(C[:c1.M] - 2.5, c1.ref) <= 2e-5
+    assert np.all(C[c1.M:] == 2.5)
+
+
+def test_plan_is_deterministic(c1):
+    C1, _ = c1.run_plan()
+    C2, _ = c1.run_plan()
+    assert np.array_equal(C1, C2)
+
+
+def test_known_answer_all_ones():
+    """The reference's own check (distSpMM_ppl_batch.cu:83,99,221 + utils.cuh:44-100):
+    B = 1 and all values 1 => C[i,:] = deg(i); exact in FP32 below 2^24."""
+    g = cola.Graph.rmat(14, 8, seed=5, gcn=False)
+    for rank in (0, 3):
+        case = Case(g, 4, rank, 32, b_ones=True)
+        lb = rank * case.P
+        deg = np.diff(case.rp)[lb:lb + case.M].astype(np.float32)
+        sag = orc.sag_unweighted(lb, lb + case.M, case.rp, case.ci, case.Bh[:case.n])
+        for C in (case.run_plan()[0], case.run_reference_path()):
+            assert orc.check_result(C[:case.M], sag) == 0
+            assert np.array_equal(C[:case.M], np.repeat(deg[:, None], 32, axis=1))
+
+
+@pytest.mark.parametrize("dim", [1, 3, 8, 32, 33, 40, 64, 96, 128, 200, 256, 512])
+def test_dims(dim):
+    """Every dim is computed (the reference skips 65..96 and > 128)."""
+    g = cola.Graph.rmat(11, 12, seed=dim)
+    case = Case(g, 2, 1, dim, seed=dim)
+    C, _ = case.run_plan(max_dim=max(dim, 256))
+    assert rel_fro(C[:case.M], case.ref) <= FP32_TOL
+    Cr = case.run_reference_path()
+    assert rel_fro(Cr[:case.M], case.ref) <= FP32_TOL
+
+
+@pytest.mark.parametrize("rank", [0, 1, 2, 3])
+def test_every_rank_and_reverse_order(rank):
+    """Ranks with peid & 2 walk remote columns in reverse (cuh:115-118)."""
+    g = cola.Graph.rmat(13, 10, seed=9, reorder=True)
+    case = Case(g, 4, rank, 64, seed=rank)
+    if case.M == 0:
+        pytest.skip("empty partition")
+    assert rel_fro(case.run_plan()[0][:case.M], case.ref) <= FP32_TOL
+    assert rel_fro(case.run_reference_path()[:case.M], case.ref) <= FP32_TOL
+
+
+def test_identity_graph():
+    """A = I: C = B_p exactly (all local, one entry per row)."""
+    n = 1000
+    g = cola.Graph.from_csr(np.arange(n + 1), np.arange(n), np.ones(n, np.float32))
+    case = Case(g, 3, 1, 64, seed=1)
+    C, _ = case.run_plan()
+    assert np.array_equal(C[:case.M], case.Bh[case.P:case.P + case.M])
+    assert np.array_equal(case.run_reference_path()[:case.M], case.Bh[case.P:case.P + case.M])
+
+
+def test_all_remote_partition():
+    """Bipartite graph: partition 0's rows only reference the other half."""
+    n = 512
+    rows, cols = [], []
+    rng = np.random.default_rng(0)
+    for i in range(256):
+        for j in rng.choice(np.arange(256, 512), 5, replace=False):
+            rows += [i, j]
+            cols += [j, i]
+    g = cola.Graph.from_coo(n, rows, cols, rng.uniform(0.5, 1.5, len(rows)).astype(np.float32))
+    case = Case(g, 2, 0, 64, seed=2)
+    assert case.info["nnz_l"] == 0 and case.info["nnz_r"] > 0
+    assert rel_fro(case.run_plan()[0][:case.M], case.ref) <= FP32_TOL
+    assert rel_fro(case.run_reference_path()[:case.M], case.ref) <= FP32_TOL
+
+
+def test_all_local_partition():
+    """nGPU = 1: everything is local, no remote columns to pack."""
+    g = cola.Graph.rmat(12, 8, seed=3)
+    case = Case(g, 1, 0, 128, seed=3)
+    assert case.info["U_r"] == 0
+    assert rel_fro(case.run_plan()[0][:case.M], case.ref) <= FP32_TOL
+    assert rel_fro(case.run_reference_path()[:case.M], case.ref) <= FP32_TOL
+
+
+def test_empty_partition():
+    """A partition without entries leaves C untouched (accumulate) / zero (overwrite)."""
+    n = 100
+    rows = np.arange(0, 40)
+    g = cola.Graph.from_coo(n, np.concatenate([rows, rows + 1]), np.concatenate([rows + 1, rows]))
+    case = Case(g, 4, 3, 32)  # rows 75..99 are isolated
+    assert case.info["nnz_l"] + case.info["nnz_r"] == 0
+    C, _ = case.run_plan(flags=cola.ACCUMULATE, init=1.0)
+    assert np.all(C == 1.0)
+    C, _ = case.run_plan(flags=cola.OVERWRITE, init=1.0)
+    assert np.all(C == 0.0)
+    assert np.all(case.run_reference_path() == 0.0)
+
+
+@pytest.mark.parametrize("hub_threshold,chunk", [(0, 0), (64, 32), (100, 100)])
+def test_hub_rows_split(hub_threshold, chunk):
+    """A star (one hub row with thousands of entries) plus a ring; hub rows are
+    cut into chunks and reduced in a fixed order."""
+    n = 6000
+    hub = np.zeros(n - 1, np.int64)
+    leaves = np.arange(1, n)
+    ring_a = np.arange(1, n)
+    ring_b = np.concatenate([np.arange(2, n), [1]])
+    rows = np.concatenate([hub, leaves, ring_a, ring_b])
+    cols = np.concatenate([leaves, hub, ring_b, ring_a])
+    vals = np.random.default_rng(1).uniform(-1, 1, len(rows)).astype(np.float32)
+    g = cola.Graph.from_coo(n, rows, cols, vals)
+    case = Case(g, 2, 0, 64, seed=4)
+    C, plan = case.run_plan(hub_threshold=hub_threshold, chunk_nnz=chunk)
+    assert plan.info["num_hub_rows"] >= 1
+    assert rel_fro(C[:case.M], case.ref) <= FP32_TOL
+    C2, _ = case.run_plan(flags=cola.ACCUMULATE | cola.KAHAN, init=0.0, hub_threshold=hub_threshold,
+                          chunk_nnz=chunk, kahan_degree=16)
+    assert rel_fro(C2[:case.M], case.ref) <= FP32_TOL
+
+
+def test_linearity_in_B(c1):
+    """A·(B1 + B2) = A·B1 + A·B2 to FP32 rounding."""
+    plan = cola.Plan(c1.part)
+    dev = c1.dev
+    B1 = c1.B
+    B2 = torch.flip(c1.B, dims=[1]).contiguous()
+    outs = []
+    for B in (B1, B2, B1 + B2):
+        packed = torch.empty((plan.U_r, c1.dim), dtype=torch.float32, device=dev)
+        C = torch.zeros((c1.P, c1.dim), dtype=torch.float32, device=dev)
+        plan.pack(B, packed)
+        plan.spmm(C, B[:c1.P], packed, flags=cola.OVERWRITE)
+        outs.append(C.double())
+    torch.cuda.synchronize()
+    d = (outs[0] + outs[1] - outs[2]).norm() / outs[2].norm()
+    assert d.item() <= 1e-5
+
+
+def test_pack_is_bitexact(c1):
+    plan = cola.Plan(c1.part)
+    packed = torch.empty((plan.U_r, c1.dim), dtype=torch.float32, device=c1.dev)
+    plan.pack(c1.B, packed)
+    a = c1.part.arrays()
+    assert np.array_equal(packed.cpu().numpy(), c1.Bh[a["csc_colIdx_r"]])
+    # unaligned / element-sized path: a dim that is not a multiple of 4
+    B3 = c1.B[:, :3].contiguous()
+    p3 = torch.empty((plan.U_r, 3), dtype=torch.float32, device=c1.dev)
+    plan.pack(B3, p3, dim=3)
+    assert np.array_equal(p3.cpu().numpy(), c1.Bh[a["csc_colIdx_r"], :3])
+
+
+def test_quantize_is_bitexact():
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, (64, 96)).astype(np.float32)
+    x[0, :8] = [0.0, -0.0, 1e-30, -1e-30, 3.0e38, 1.0, 1.0 + 2**-8, 1.0 + 3 * 2**-8]
+    x[1] = 0.0
+    xt = torch.from_numpy(x).cuda()
+    bf, _ = cola.quantize_rows(xt, cola.BF16)
+    assert np.array_equal(bf.cpu().numpy().view(np.uint16), orc.bf16_from_f32(x))
+    f8, sc = cola.quantize_rows(xt, cola.FP8_E4M3)
+    sc = sc.cpu().numpy()
+    exp_sc = np.array([orc.fp8_row_scale(r) for r in x], np.float32)
+    assert np.array_equal(sc, exp_sc)
+    exp_codes = orc.fp8_from_f32((x / exp_sc[:, None]).astype(np.float32))
+    assert np.array_equal(f8.cpu().numpy(), exp_codes)
+    h, _ = cola.quantize_rows(xt, cola.F16)
+    assert np.array_equal(h.cpu().numpy().view(np.float16), x.astype(np.float16))
+
+
+# Error bounds of the reduced-precision storage forms against the FP64 oracle
+# (relative Frobenius): unit roundoff 2^-9 (bf16), 2^-12 (fp16), 2^-4 (e4m3)
+# per stored element, averaged down by the row sums.
+PREC_TOL = {cola.BF16: 4e-3, cola.F16: 5e-4, cola.FP8_E4M3: 5e-2}
+
+
+@pytest.mark.parametrize("dtype", [cola.BF16, cola.F16, cola.FP8_E4M3])
+@pytest.mark.parametrize("dim", [64, 128, 256])
+def test_precision_forms(c1, dtype, dim):
+    rng = np.random.default_rng(dim)
+    Bh = rng.uniform(-1, 1, (c1.P * c1.nGPU, dim)).astype(np.float32)
+    ref = orc.spmm_f64(c1.rp, c1.ci, c1.va, 0, c1.M, Bh[:c1.n])
+    B = torch.from_numpy(Bh).cuda()
+    Bq, sc = cola.quantize_rows(B, dtype)
+    plan = cola.Plan(c1.part)
+    eb = cola.DTYPE_BYTES[dtype]
+    tdt = {1: torch.uint8, 2: torch.uint16}[eb]
+    packed = torch.empty((plan.U_r, dim), dtype=tdt, device=B.device)
+    psc = torch.empty(plan.U_r, dtype=torch.float32, device=B.device) if dtype == cola.FP8_E4M3 else None
+    plan.pack(Bq, packed, dim=dim, dtype=dtype, scales_global=sc, packed_scales=psc)
+    C = torch.zeros((c1.P, dim), dtype=torch.float32, device=B.device)
+    plan.spmm(C, Bq[:c1.P], packed, dim=dim, dtype=dtype, flags=cola.OVERWRITE | cola.KAHAN,
+              scales_local=None if sc is None else sc[:c1.P], packed_scales=psc)
+    torch.cuda.synchronize()
+    err = rel_fro(C.cpu().numpy()[:c1.M], ref)
+    # Also: the device result equals the FP64 product of the dequantised B.
+    if dtype == cola.FP8_E4M3:
+        deq = orc.fp8_to_f32(Bq.cpu().numpy()) * sc.cpu().numpy()[:, None]
+    elif dtype == cola.BF16:
+        deq = (Bq.cpu().numpy().astype(np.uint32) << 16).view(np.float32)
+    else:
+        deq = Bq.cpu().numpy().view(np.float16).astype(np.float32)
+    ref_q = orc.spmm_f64(c1.rp, c1.ci, c1.va, 0, c1.M, deq[:c1.n])
+    err_q = rel_fro(C.cpu().numpy()[:c1.M], ref_q)
+    print(f"dtype={cola.DTYPE_NAMES[dtype]} dim={dim} rel_fro_vs_fp64={err:.3e} vs_dequantised={err_q:.3e}")
+    assert err <= PREC_TOL[dtype]
+    assert err_q <= FP32_TOL
+
+
+def test_error_norms_on_device(c1):
+    C, _ = c1.run_plan()
+    Ct = torch.from_numpy(C[:c1.M]).cuda()
+    Rt = torch.from_numpy(c1.ref).cuda()
+    out = cola.error_norms(Ct, Rt).cpu().numpy()
+    d = C[:c1.M].astype(np.float64) - c1.ref
+    assert np.isclose(out[0], (d * d).sum(), rtol=1e-10, atol=1e-300)
+    assert np.isclose(out[1], (c1.ref ** 2).sum(), rtol=1e-12)
+    rn = (c1.ref ** 2).sum(1)
+    nz = rn > 0
+    assert np.isclose(out[2], ((d * d).sum(1)[nz] / rn[nz]).max(), rtol=1e-10)
+
+
+def test_full_size_partition_checksum():
+    """The 2^22-node partition (R-MAT, degree 32, Cuthill-McKee order,
+    partition 0 of 8, dim 128): row sums of C equal A_p·(B·1) computed by the
+    FP64 oracle (a size-independent checksum), and a sample of rows matches
+    the FP64 product."""
+    g = cola.Graph.rmat(22, 32, seed=2, reorder=True)
+    rp, ci, va = g.arrays()
+    part = g.partition(8, 0)
+    info = part.info()
+    P, M, dim = info["nodesPerPE"], info["num_rows"], 128
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    B = torch.rand((P * 8, dim), generator=gen, device="cuda", dtype=torch.float32) * 2 - 1
+    plan = cola.Plan(part)
+    packed = torch.empty((plan.U_r, dim), dtype=torch.float32, device="cuda")
+    C = torch.zeros((P, dim), dtype=torch.float32, device="cuda")
+    plan.pack(B, packed)
+    plan.spmm(C, B[:P], packed, flags=cola.OVERWRITE)
+    rowsum_B = B.double().sum(1).cpu().numpy()
+    rowsum_C = C.double().sum(1).cpu().numpy()[:M]
+    torch.cuda.synchronize()
+    ref = orc.spmm_f64(rp, ci, va, 0, M, rowsum_B.astype(np.float32)[:, None])[:, 0]
+    assert np.linalg.norm(rowsum_C - ref) / np.linalg.norm(ref) <= 1e-5
+    sample = np.unique(np.concatenate([np.arange(0, M, 997), [0, 1, M - 1]]))
+    Bh = B.cpu().numpy()
+    Ch = C.cpu().numpy()
+    refs = np.stack([orc.spmm_f64(rp, ci, va, int(i), int(i) + 1, Bh[:g.n])[0] for i in sample])
+    assert rel_fro(Ch[sample], refs) <= FP32_TOL
