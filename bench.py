@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--copy-calibrate-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one partition on one GPU; runs on more than one GPU are not part of "
@@ -76,6 +78,19 @@ def main():
     B_local = B[args.rank * P:(args.rank + 1) * P]
     packed = torch.empty((max(U_r, 1), N), dtype=torch.float32, device=dev)
     C = torch.empty((P, N), dtype=torch.float32, device=dev)
+
+    if args.copy_calibrate_mib > 0:
+        nel = args.copy_calibrate_mib * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    plan.pack(B, packed)
+    tuning, tune_ms = plan.autotune(C, B_local, packed, iters=3)
+    log(f"autotuned f32: {tuning} ({tune_ms:.3f} ms)")
 
     def step():
         plan.pack(B, packed)
@@ -129,33 +144,46 @@ def main():
         err = float(np.linalg.norm(Ch[sample] - ref) / np.linalg.norm(ref))
         extras["fp32_rel_fro_sample"] = err
         log(f"fp32 sample rel_fro {err:.2e}")
-        # BF16 storage of B (FP32 accumulation, Kahan on hub rows).
-        Bq, _ = cola.quantize_rows(B, cola.BF16)
-        pq = torch.empty((max(U_r, 1), N), dtype=torch.uint16, device=dev)
-        Cq = torch.empty_like(C)
+        # Reduced-precision storage of B (FP32 accumulation, Kahan on hub rows),
+        # each with its own autotuned schedule.
+        f32_tuning = plan.tuning
+        for dt, tdt in ((cola.BF16, torch.uint16), (cola.FP8_E4M3, torch.uint8)):
+            name = cola.DTYPE_NAMES[dt]
+            Bq, sc = cola.quantize_rows(B, dt)
+            pq = torch.empty((max(U_r, 1), N), dtype=tdt, device=dev)
+            psc = torch.empty(max(U_r, 1), dtype=torch.float32, device=dev) if dt == cola.FP8_E4M3 else None
+            Bq_l = Bq[args.rank * P:(args.rank + 1) * P]
+            sc_l = None if sc is None else sc[args.rank * P:(args.rank + 1) * P]
+            Cq = torch.empty_like(C)
+            plan.pack(Bq, pq, dtype=dt, scales_global=sc, packed_scales=psc)
+            tq, _ = plan.autotune(Cq, Bq_l, pq, dtype=dt, flags=cola.OVERWRITE | cola.KAHAN, iters=3,
+                                  scales_local=sc_l, packed_scales=psc)
 
-        def step_bf16():
-            plan.pack(Bq, pq, dtype=cola.BF16)
-            plan.spmm(Cq, Bq[args.rank * P:(args.rank + 1) * P], pq, dtype=cola.BF16,
-                      flags=cola.OVERWRITE | cola.KAHAN)
+            def step_q():
+                plan.pack(Bq, pq, dtype=dt, scales_global=sc, packed_scales=psc)
+                plan.spmm(Cq, Bq_l, pq, dtype=dt, flags=cola.OVERWRITE | cola.KAHAN, scales_local=sc_l,
+                          packed_scales=psc)
 
-        for _ in range(args.warmup):
-            step_bf16()
-        torch.cuda.synchronize()
-        s2, e2 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s2.record()
-        for _ in range(args.steps):
-            step_bf16()
-        e2.record()
-        torch.cuda.synchronize()
-        ms_bf16 = s2.elapsed_time(e2) / args.steps
-        Cqh = Cq.cpu().numpy()
-        err_bf16 = float(np.linalg.norm(Cqh[sample] - ref) / np.linalg.norm(ref))
-        extras["bf16"] = {"ms_per_step": ms_bf16,
-                          "GBps_alg": bytes_alg(M, nnz_p, U_l + U_r, N, 2) / (ms_bf16 * 1e-3) / 1e9,
-                          "speedup_vs_fp32": ms / ms_bf16, "rel_fro_sample_vs_fp64": err_bf16}
-        log(f"bf16 step {ms_bf16:.3f} ms ({ms / ms_bf16:.2f}x), rel_fro {err_bf16:.2e}")
-        del Bq, pq, Cq
+            for _ in range(args.warmup):
+                step_q()
+            torch.cuda.synchronize()
+            s2, e2 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            s2.record()
+            for _ in range(args.steps):
+                step_q()
+            e2.record()
+            torch.cuda.synchronize()
+            ms_q = s2.elapsed_time(e2) / args.steps
+            Cqh = Cq.cpu().numpy()
+            err_q = float(np.linalg.norm(Cqh[sample] - ref) / np.linalg.norm(ref))
+            s_sc = 4 if dt == cola.FP8_E4M3 else 0
+            extras[name] = {"ms_per_step": ms_q,
+                            "GBps_alg": bytes_alg(M, nnz_p, U_l + U_r, N, cola.DTYPE_BYTES[dt], s_sc) /
+                            (ms_q * 1e-3) / 1e9,
+                            "speedup_vs_fp32": ms / ms_q, "rel_fro_sample_vs_fp64": err_q, "tuning": tq}
+            log(f"{name} step {ms_q:.3f} ms ({ms / ms_q:.2f}x fp32), rel_fro {err_q:.2e}, tuning {tq}")
+            del Bq, pq, Cq, sc, psc
+        plan.tuning = f32_tuning
         # Reference-semantics path: nnz_split virtual rows + CSC scatter atomics.
         a = part.arrays()
         tl, pl, _ = cola.nnz_split(a["csr_rowIdx_l"], a["csr_rowPtr_l"], len(a["csr_colIdx_l"]))
@@ -228,7 +256,8 @@ def main():
                                f"_dim{N}_f32",
                    "nodes": n, "nnz_graph": nnz_total, "rows": M, "nnz_partition": nnz_p,
                    "unique_local_cols": U_l, "unique_remote_cols": U_r, "dim": N,
-                   "bytes_alg": balg, "step": "pack + plan row kernel + hub reduce, C overwritten"},
+                   "bytes_alg": balg, "step": "pack + plan row kernel + hub reduce, C overwritten",
+                   "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": round(k_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "plan_rows_kernel (+ hub_reduce_kernel)", "kernel_ms": round(spmm_ms, 4),

@@ -52,13 +52,18 @@ class PartitionArrays(ctypes.Structure):
 
 class PlanOpts(ctypes.Structure):
     _fields_ = [("hub_threshold", c_i32), ("chunk_nnz", c_i32), ("kahan_degree", c_i32),
-                ("max_dim", c_i32)]
+                ("max_dim", c_i32), ("short_degree", c_i32)]
+
+
+class PlanTuning(ctypes.Structure):
+    _fields_ = [("col_tile", c_i32), ("rpw_short", c_i32), ("rpw_long", c_i32), ("blocks_per_cu", c_i32),
+                ("xcd_remap", c_i32)]
 
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [("num_rows", c_i64), ("num_items", c_i64), ("num_hub_rows", c_i64),
                 ("num_hub_chunks", c_i64), ("nnz", c_i64), ("U_r", c_i64), ("U_l", c_i64),
-                ("device_bytes", c_i64)]
+                ("device_bytes", c_i64), ("num_short_items", c_i64), ("num_long_items", c_i64)]
 
 
 # name -> argtypes (all return cola_status unless listed in _RESTYPES)
@@ -90,6 +95,10 @@ SIGNATURES = {
     "cola_plan_create": [c_vp, ctypes.POINTER(PlanOpts), ctypes.POINTER(c_vp)],
     "cola_plan_get_info": [c_vp, ctypes.POINTER(PlanInfo)],
     "cola_plan_remote_cols": [c_vp, ctypes.POINTER(c_vp)],
+    "cola_plan_set_tuning": [c_vp, ctypes.POINTER(PlanTuning)],
+    "cola_plan_get_tuning": [c_vp, ctypes.POINTER(PlanTuning)],
+    "cola_plan_autotune": [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_int, c_int, c_int, c_int,
+                           c_vp, ctypes.POINTER(PlanTuning), ctypes.POINTER(ctypes.c_float)],
     "cola_plan_destroy": [c_vp],
     "cola_spmm_partition": [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_int, c_int,
                             c_int, c_vp],

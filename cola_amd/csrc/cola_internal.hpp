@@ -49,9 +49,12 @@ struct HostPlan {
   // remote row ~col.
   std::vector<int32_t> cols;
   std::vector<float> vals;
-  std::vector<WorkItem> items;       // non-empty rows and hub chunks
+  // Degree bins: short rows (at most short_degree entries) and long rows
+  // (the rest, hub rows as chunks), each in row order with a contiguous item
+  // range per XCD (9 entries).
+  std::vector<WorkItem> items_short, items_long;
+  std::vector<int32_t> xcd_short, xcd_long;
   std::vector<int32_t> empty_rows;   // rows without entries (OVERWRITE)
-  std::vector<int32_t> xcd_begin;    // 9 entries: item range per XCD
   std::vector<int32_t> hub_rows;     // output row of each hub
   std::vector<int32_t> hub_chunk_ptr;// chunk range per hub (num_hubs+1)
   std::vector<int32_t> remote_cols;  // unique remote global columns (U_r)

@@ -459,7 +459,9 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
   }
 
   // Work items in row order; hub rows become chunk items.
-  out->items.clear();
+  const int32_t sdeg = o.short_degree > 0 ? o.short_degree : 32;
+  out->items_short.clear();
+  out->items_long.clear();
   out->empty_rows.clear();
   out->hub_rows.clear();
   out->hub_chunk_ptr.assign(1, 0);
@@ -471,12 +473,14 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
       continue;
     }
     const int32_t kflag = d > kdeg ? 1 : 0;
-    if (d <= hub) {
-      out->items.push_back({(int32_t)i, (int32_t)b, (int32_t)e, kflag});
+    if (d <= sdeg) {
+      out->items_short.push_back({(int32_t)i, (int32_t)b, (int32_t)e, kflag});
+    } else if (d <= hub) {
+      out->items_long.push_back({(int32_t)i, (int32_t)b, (int32_t)e, kflag});
     } else {
       for (int64_t s = b; s < e; s += chunk) {
-        out->items.push_back({-(nchunks + 1), (int32_t)s,
-                              (int32_t)std::min<int64_t>(e, s + chunk), kflag});
+        out->items_long.push_back({-(nchunks + 1), (int32_t)s,
+                                   (int32_t)std::min<int64_t>(e, s + chunk), kflag});
         ++nchunks;
       }
       out->hub_rows.push_back((int32_t)i);
@@ -484,20 +488,22 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
     }
   }
   // Contiguous item ranges per XCD, balanced by entries + a per-item cost.
-  const int kXcd = 8;
-  const int64_t kItemCost = 8;
-  int64_t total = 0;
-  for (const auto &it : out->items) total += (it.end - it.begin) + kItemCost;
-  out->xcd_begin.assign(kXcd + 1, (int32_t)out->items.size());
-  out->xcd_begin[0] = 0;
-  {
+  auto split = [](const std::vector<WorkItem> &items, std::vector<int32_t> *xb) {
+    const int kXcd = 8;
+    const int64_t kItemCost = 8;
+    int64_t total = 0;
+    for (const auto &it : items) total += (it.end - it.begin) + kItemCost;
+    xb->assign(kXcd + 1, (int32_t)items.size());
+    (*xb)[0] = 0;
     int64_t acc = 0;
     int x = 1;
-    for (size_t k = 0; k < out->items.size() && x < kXcd; ++k) {
-      while (x < kXcd && acc >= (total * x) / kXcd) out->xcd_begin[x++] = (int32_t)k;
-      acc += (out->items[k].end - out->items[k].begin) + kItemCost;
+    for (size_t k = 0; k < items.size() && x < kXcd; ++k) {
+      while (x < kXcd && acc >= (total * x) / kXcd) (*xb)[x++] = (int32_t)k;
+      acc += (items[k].end - items[k].begin) + kItemCost;
     }
-  }
+  };
+  split(out->items_short, &out->xcd_short);
+  split(out->items_long, &out->xcd_long);
   return COLA_OK;
 }
 

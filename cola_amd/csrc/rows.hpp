@@ -1,0 +1,367 @@
+// Row kernel of the plan path, shared by the per-dtype translation units
+// (rows_f32.hip, rows_bf16.hip, rows_f16.hip, rows_fp8.hip).
+//
+// A work item is one output row (or one chunk of a hub row) whose entries
+// address either a row of this partition's B slice (entry column >= 0) or a
+// row of the packed remote buffer (entry column < 0, packed row ~column).
+//
+// Lane layout (wave64): the dense dimension of one B row is spread over a
+// row group of G lanes, EPL consecutive elements per lane per 16-byte vector
+// (EPL = 4 FP32, 8 BF16/FP16, 16 FP8), NV vectors per lane.  The wave's
+// S = 64/G groups form `rpw` teams; a team works on one item, its groups
+// taking every (S/rpw)-th entry, U loads in flight per group, and the groups
+// of a team are summed with cross-lane shuffles at the end.  rpw = 1 is one
+// item per wave (long rows); rpw = S is one item per group (short rows).
+// Entry (column, value) pairs are fetched with one coalesced load per team
+// and per 64/rpw entries and broadcast with ds_bpermute.
+#pragma once
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/cola_spmm.h"
+
+namespace cola_dev {
+
+__device__ __forceinline__ float bf16_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
+__device__ __forceinline__ float f16_to_f32(uint32_t h) {
+  return __half2float(__ushort_as_half((unsigned short)h));
+}
+// OCP FP8 E4M3 (e4m3fn): 1 sign, 4 exponent (bias 7), 3 mantissa bits;
+// exponent 0 is subnormal (m * 2^-9); 0x7f / 0xff are NaN (never written).
+__device__ __forceinline__ float fp8e4m3_to_f32(uint32_t c) {
+  const uint32_t e = (c >> 3) & 15u, m = c & 7u;
+  const float mag = e == 0 ? (float)m * 0x1p-9f : __uint_as_float(((c & 0x7fu) << 20) + (120u << 23));
+  return (c & 0x80u) ? -mag : mag;
+}
+
+// Vec<DT, EPL>: element type T, the raw register form of one lane's vector,
+// its load and its widening to FP32.
+template <int DT, int EPL>
+struct Vec;
+template <>
+struct Vec<COLA_F32, 4> {
+  using T = float;
+  using Raw = float4;
+  static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const float4 *>(p); }
+  static __device__ __forceinline__ void widen(const Raw &r, float (&x)[4]) {
+    x[0] = r.x; x[1] = r.y; x[2] = r.z; x[3] = r.w;
+  }
+};
+template <>
+struct Vec<COLA_F32, 1> {
+  using T = float;
+  using Raw = float;
+  static __device__ __forceinline__ Raw load(const T *p) { return *p; }
+  static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = r; }
+};
+template <int DT>
+struct Vec16 {
+  using T = uint16_t;
+  using Raw = uint4;
+  static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const uint4 *>(p); }
+  static __device__ __forceinline__ float cvt(uint32_t h) {
+    if constexpr (DT == COLA_BF16) return bf16_to_f32(h);
+    else return f16_to_f32(h);
+  }
+  static __device__ __forceinline__ void widen(const Raw &r, float (&x)[8]) {
+    const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      x[2 * i] = cvt(w[i] & 0xffffu);
+      x[2 * i + 1] = cvt(w[i] >> 16);
+    }
+  }
+};
+template <>
+struct Vec<COLA_BF16, 8> : Vec16<COLA_BF16> {};
+template <>
+struct Vec<COLA_F16, 8> : Vec16<COLA_F16> {};
+template <int DT>
+struct Vec16x1 {
+  using T = uint16_t;
+  using Raw = uint32_t;
+  static __device__ __forceinline__ Raw load(const T *p) { return *p; }
+  static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = Vec16<DT>::cvt(r); }
+};
+template <>
+struct Vec<COLA_BF16, 1> : Vec16x1<COLA_BF16> {};
+template <>
+struct Vec<COLA_F16, 1> : Vec16x1<COLA_F16> {};
+template <>
+struct Vec<COLA_FP8_E4M3, 16> {
+  using T = uint8_t;
+  using Raw = uint4;
+  static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const uint4 *>(p); }
+  static __device__ __forceinline__ void widen(const Raw &r, float (&x)[16]) {
+    const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) x[4 * i + b] = fp8e4m3_to_f32((w[i] >> (8 * b)) & 0xffu);
+  }
+};
+template <>
+struct Vec<COLA_FP8_E4M3, 1> {
+  using T = uint8_t;
+  using Raw = uint32_t;
+  static __device__ __forceinline__ Raw load(const T *p) { return *p; }
+  static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = fp8e4m3_to_f32(r); }
+};
+
+struct RowArgs {
+  const int4 *items;
+  const int32_t *xcd_begin;  // 9 entries
+  const int32_t *cols;
+  const float *vals;
+  const void *Bl;
+  int64_t ldB;
+  const float *sl;
+  const void *Bp;
+  int64_t ldP;
+  const float *sp;
+  float *C;
+  int64_t ldC;
+  float *partials;
+  int64_t pstride;
+  int dim;
+  int accumulate;
+  int kahan;
+  int rpw;        // teams (items in flight) per wave, power of two <= 64/G
+  int xcd_remap;  // 1: block b walks XCD (b % 8)'s item range
+};
+
+template <int DT, int EPL, int G, int NV, int U, bool KAHAN>
+__global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
+  using V = Vec<DT, EPL>;
+  using T = typename V::T;
+  using Raw = typename V::Raw;
+  constexpr int S = 64 / G;
+  const T *__restrict__ Bl = static_cast<const T *>(a.Bl);
+  const T *__restrict__ Bp = static_cast<const T *>(a.Bp);
+  const int lane = threadIdx.x & 63;
+  const int sub = lane / G, gl = lane % G;
+  const int rpw = a.rpw;
+  const int gpt = S / rpw;     // groups per team
+  const int tsize = 64 / rpw;  // lanes per team
+  const int team = sub / gpt, tsub = sub % gpt;
+  const int tbase = team * tsize, tl = lane - tbase;
+  const int wpb = blockDim.x >> 6;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int blk, nblk, ib, ie;
+  if (a.xcd_remap) {
+    const int xcd = blockIdx.x & 7;
+    blk = blockIdx.x >> 3;
+    nblk = (gridDim.x - xcd + 7) >> 3;
+    ib = a.xcd_begin[xcd];
+    ie = a.xcd_begin[xcd + 1];
+  } else {
+    blk = blockIdx.x;
+    nblk = gridDim.x;
+    ib = a.xcd_begin[0];
+    ie = a.xcd_begin[8];
+  }
+  const int wave = blk * wpb + wid;
+  const int nwave = nblk * wpb;
+
+  for (int it0 = ib + wave * rpw; it0 < ie; it0 += nwave * rpw) {
+    const int it = it0 + team;
+    const bool has = it < ie;
+    int4 item = make_int4(0, 0, 0, 0);
+    if (has) item = a.items[it];
+    const int b = item.y, e = item.z;
+    // Compensated sums only in the KAHAN instantiation and on flagged items.
+    const bool comp = KAHAN && (item.w & 1);
+    float acc[NV][EPL], err[NV][EPL];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) acc[k][q] = err[k][q] = 0.f;
+
+    for (int base = b; base < e; base += tsize) {
+      const int n = min(tsize, e - base);
+      int mc = 0;
+      float mv = 0.f;
+      if (tl < n) {
+        mc = a.cols[base + tl];
+        mv = a.vals[base + tl];
+      }
+      for (int j = 0; j < n; j += gpt * U) {
+        Raw raw[U][NV];
+        float av[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int ei = j + u * gpt + tsub;
+          const bool ok = ei < n;
+          const int src = (tbase + (ok ? ei : 0)) << 2;
+          const int c = __builtin_amdgcn_ds_bpermute(src, mc);
+          float v = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv)));
+          const T *rp = c >= 0 ? Bl + (int64_t)c * a.ldB : Bp + (int64_t)(~c) * a.ldP;
+          if constexpr (DT == COLA_FP8_E4M3) {
+            if (ok) v *= (c >= 0 ? a.sl[c] : a.sp[~c]);
+          }
+          av[u] = ok ? v : 0.f;
+#pragma unroll
+          for (int k = 0; k < NV; ++k) {
+            const int col = (k * G + gl) * EPL;
+            if (ok && col < a.dim) raw[u][k] = V::load(rp + col);
+            else raw[u][k] = Raw{};
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int k = 0; k < NV; ++k) {
+            float x[EPL];
+            V::widen(raw[u][k], x);
+            if (!comp) {
+#pragma unroll
+              for (int q = 0; q < EPL; ++q) acc[k][q] = fmaf(av[u], x[q], acc[k][q]);
+            } else {
+              // Kahan: err carries the low-order part lost by acc.
+#pragma unroll
+              for (int q = 0; q < EPL; ++q) {
+                const float y = fmaf(av[u], x[q], -err[k][q]);
+                const float t = acc[k][q] + y;
+                err[k][q] = (t - acc[k][q]) - y;
+                acc[k][q] = t;
+              }
+            }
+          }
+      }
+    }
+    // Sum the groups of each team (xor offsets below tsize stay in the team).
+    for (int off = G; off < tsize; off <<= 1) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) {
+          const float s2 = __shfl_xor(acc[k][q], off);
+          const float e2 = __shfl_xor(err[k][q], off);
+          if (!comp) {
+            acc[k][q] += s2;
+          } else {
+            // Neumaier combination of (acc, -err) pairs.
+            const float s1 = acc[k][q];
+            const float t = s1 + s2;
+            const float lost = fabsf(s1) >= fabsf(s2) ? (s1 - t) + s2 : (s2 - t) + s1;
+            acc[k][q] = t;
+            err[k][q] = err[k][q] + e2 - lost;
+          }
+        }
+    }
+    if (has && tsub == 0) {
+      float *dst;
+      bool add_old;
+      if (item.x >= 0) {
+        dst = a.C + (int64_t)item.x * a.ldC;
+        add_old = a.accumulate != 0;
+      } else {
+        dst = a.partials + (int64_t)(-item.x - 1) * a.pstride;
+        add_old = false;
+      }
+#pragma unroll
+      for (int k = 0; k < NV; ++k) {
+        const int col = (k * G + gl) * EPL;
+        if (col < a.dim) {
+          float v[EPL];
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) v[q] = comp ? acc[k][q] - err[k][q] : acc[k][q];
+          if constexpr (EPL % 4 == 0) {
+#pragma unroll
+            for (int q = 0; q < EPL; q += 4) {
+              float4 *d4 = reinterpret_cast<float4 *>(dst + col + q);
+              float4 o = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
+              if (add_old) {
+                const float4 old = *d4;
+                o.x += old.x; o.y += old.y; o.z += old.z; o.w += old.w;
+              }
+              *d4 = o;
+            }
+          } else {
+#pragma unroll
+            for (int q = 0; q < EPL; ++q) dst[col + q] = add_old ? dst[col + q] + v[q] : v[q];
+          }
+        }
+      }
+    }
+  }
+}
+
+// Lanes per row group and vectors per lane for a dim.  vec_ok: dim, leading
+// dimensions and base pointers allow 16-byte vectors.
+struct Shape {
+  int epl, g, nv;
+};
+
+inline bool pick_shape(int dt, int dim, bool vec_ok, Shape *s) {
+  if (dim < 1) return false;
+  const int ev = dt == COLA_F32 ? 4 : (dt == COLA_FP8_E4M3 ? 16 : 8);
+  if (vec_ok && dim % ev == 0) {
+    const int chunks = dim / ev;
+    s->epl = ev;
+    s->nv = 1;
+    if (chunks <= 2) s->g = 2;
+    else if (chunks <= 4) s->g = 4;
+    else if (chunks <= 8) s->g = 8;
+    else if (chunks <= 16) s->g = 16;
+    else if (chunks <= 32) s->g = 32;
+    else if (chunks <= 64) s->g = 64;
+    else if (chunks <= 128) { s->g = 64; s->nv = 2; }
+    else if (chunks <= 256) { s->g = 64; s->nv = 4; }
+    else return false;
+    return true;
+  }
+  s->epl = 1;
+  s->g = 64;
+  if (dim <= 64) s->nv = 1;
+  else if (dim <= 128) s->nv = 2;
+  else if (dim <= 256) s->nv = 4;
+  else if (dim <= 512) s->nv = 8;
+  else return false;
+  return true;
+}
+
+// Loads in flight per group: 8, fewer when a lane already holds several
+// vectors per entry.
+template <int NV>
+struct Unroll {
+  static constexpr int value = NV >= 8 ? 1 : 8 / NV;
+};
+
+// Launches one row kernel; returns false if no instantiation matches.
+bool launch_rows_f32(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s);
+bool launch_rows_bf16(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s);
+bool launch_rows_f16(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s);
+bool launch_rows_fp8(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s);
+
+template <int DT, int EV>
+bool launch_rows_impl(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s) {
+#define COLA_ROWS_CASE(E, G_, NV_)                                                                \
+  if (sh.epl == E && sh.g == G_ && sh.nv == NV_) {                                                \
+    if (a.kahan)                                                                                  \
+      hipLaunchKernelGGL((plan_rows_kernel<DT, E, G_, NV_, Unroll<NV_>::value, true>), dim3(blocks), \
+                         dim3(threads), 0, s, a);                                                 \
+    else                                                                                          \
+      hipLaunchKernelGGL((plan_rows_kernel<DT, E, G_, NV_, Unroll<NV_>::value, false>),           \
+                         dim3(blocks), dim3(threads), 0, s, a);                                   \
+    return true;                                                                                  \
+  }
+  COLA_ROWS_CASE(EV, 2, 1)
+  COLA_ROWS_CASE(EV, 4, 1)
+  COLA_ROWS_CASE(EV, 8, 1)
+  COLA_ROWS_CASE(EV, 16, 1)
+  COLA_ROWS_CASE(EV, 32, 1)
+  COLA_ROWS_CASE(EV, 64, 1)
+  COLA_ROWS_CASE(EV, 64, 2)
+  COLA_ROWS_CASE(EV, 64, 4)
+  COLA_ROWS_CASE(1, 64, 1)
+  COLA_ROWS_CASE(1, 64, 2)
+  COLA_ROWS_CASE(1, 64, 4)
+  COLA_ROWS_CASE(1, 64, 8)
+#undef COLA_ROWS_CASE
+  return false;
+}
+
+}  // namespace cola_dev

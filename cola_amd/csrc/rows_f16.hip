@@ -1,0 +1,10 @@
+// Row-kernel instantiations for B stored as f16 (EPL = 8 elements per
+// 16-byte lane vector).  One translation unit per storage type so the
+// instantiations compile in parallel.
+#include "rows.hpp"
+
+namespace cola_dev {
+bool launch_rows_f16(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s) {
+  return launch_rows_impl<COLA_F16, 8>(sh, a, blocks, threads, s);
+}
+}  // namespace cola_dev

@@ -1,0 +1,10 @@
+// Row-kernel instantiations for B stored as fp8 (EPL = 16 elements per
+// 16-byte lane vector).  One translation unit per storage type so the
+// instantiations compile in parallel.
+#include "rows.hpp"
+
+namespace cola_dev {
+bool launch_rows_fp8(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s) {
+  return launch_rows_impl<COLA_FP8_E4M3, 16>(sh, a, blocks, threads, s);
+}
+}  // namespace cola_dev

@@ -24,6 +24,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -31,35 +33,24 @@
 #include <vector>
 
 #include "cola_internal.hpp"
+#include "rows.hpp"
 
 using cola::fail;
 
 namespace {
 
-// ---------------------------------------------------------------------------
-// Element formats.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ float bf16_to_f32(uint32_t h) {
-  return __uint_as_float(h << 16);
-}
-__device__ __forceinline__ float f16_to_f32(uint32_t h) {
-  return __half2float(__ushort_as_half((unsigned short)h));
-}
-// OCP FP8 E4M3 (e4m3fn): 1 sign, 4 exponent (bias 7), 3 mantissa bits; code
-// with exponent 0 is subnormal (m * 2^-9); 0x7f/0xff are NaN (never written).
-__device__ __forceinline__ float fp8e4m3_to_f32(uint32_t c) {
-  const uint32_t e = (c >> 3) & 15u, m = c & 7u;
-  const float mag = e == 0 ? (float)m * 0x1p-9f
-                           : __uint_as_float(((c & 0x7fu) << 20) + (120u << 23));
-  return (c & 0x80u) ? -mag : mag;
-}
+using cola_dev::Shape;
+using cola_dev::Vec;
+using cola_dev::fp8e4m3_to_f32;
+
+// Round-to-nearest-even into bfloat16 (NaN -> quiet NaN with its sign).
 __device__ __forceinline__ uint32_t f32_to_bf16_rne(float f) {
   uint32_t u = __float_as_uint(f);
   if ((u & 0x7fffffffu) > 0x7f800000u) return ((u >> 16) & 0x8000u) | 0x7fc0u;
   u += 0x7fffu + ((u >> 16) & 1u);
   return u >> 16;
 }
-// Round-to-nearest-even into e4m3fn, saturating at +-448 (finite inputs).
+// Round-to-nearest-even into OCP FP8 E4M3 (e4m3fn), saturating at +-448.
 __device__ __forceinline__ uint32_t f32_to_fp8e4m3_rne(float f) {
   const uint32_t sign = (__float_as_uint(f) >> 24) & 0x80u;
   float a = fabsf(f);
@@ -75,66 +66,6 @@ __device__ __forceinline__ uint32_t f32_to_fp8e4m3_rne(float f) {
   if (e > 15 || (e == 15 && m == 7)) return sign | 0x7eu;
   return sign | ((uint32_t)e << 3) | m;
 }
-
-// Loads EPL consecutive elements (EPL = 4: one 16/8/4-byte vector; EPL = 1:
-// one scalar) of a row and widens them to FP32.
-template <int DT, int EPL>
-struct Row;
-template <>
-struct Row<COLA_F32, 4> {
-  using T = float;
-  static __device__ __forceinline__ void load(const T *p, float (&x)[4]) {
-    const float4 v = *reinterpret_cast<const float4 *>(p);
-    x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
-  }
-};
-template <>
-struct Row<COLA_F32, 1> {
-  using T = float;
-  static __device__ __forceinline__ void load(const T *p, float (&x)[1]) { x[0] = *p; }
-};
-template <>
-struct Row<COLA_BF16, 4> {
-  using T = uint16_t;
-  static __device__ __forceinline__ void load(const T *p, float (&x)[4]) {
-    const uint2 v = *reinterpret_cast<const uint2 *>(p);
-    x[0] = bf16_to_f32(v.x & 0xffffu); x[1] = bf16_to_f32(v.x >> 16);
-    x[2] = bf16_to_f32(v.y & 0xffffu); x[3] = bf16_to_f32(v.y >> 16);
-  }
-};
-template <>
-struct Row<COLA_BF16, 1> {
-  using T = uint16_t;
-  static __device__ __forceinline__ void load(const T *p, float (&x)[1]) { x[0] = bf16_to_f32(*p); }
-};
-template <>
-struct Row<COLA_F16, 4> {
-  using T = uint16_t;
-  static __device__ __forceinline__ void load(const T *p, float (&x)[4]) {
-    const uint2 v = *reinterpret_cast<const uint2 *>(p);
-    x[0] = f16_to_f32(v.x & 0xffffu); x[1] = f16_to_f32(v.x >> 16);
-    x[2] = f16_to_f32(v.y & 0xffffu); x[3] = f16_to_f32(v.y >> 16);
-  }
-};
-template <>
-struct Row<COLA_F16, 1> {
-  using T = uint16_t;
-  static __device__ __forceinline__ void load(const T *p, float (&x)[1]) { x[0] = f16_to_f32(*p); }
-};
-template <>
-struct Row<COLA_FP8_E4M3, 4> {
-  using T = uint8_t;
-  static __device__ __forceinline__ void load(const T *p, float (&x)[4]) {
-    const uint32_t v = *reinterpret_cast<const uint32_t *>(p);
-    x[0] = fp8e4m3_to_f32(v & 0xffu); x[1] = fp8e4m3_to_f32((v >> 8) & 0xffu);
-    x[2] = fp8e4m3_to_f32((v >> 16) & 0xffu); x[3] = fp8e4m3_to_f32(v >> 24);
-  }
-};
-template <>
-struct Row<COLA_FP8_E4M3, 1> {
-  using T = uint8_t;
-  static __device__ __forceinline__ void load(const T *p, float (&x)[1]) { x[0] = fp8e4m3_to_f32(*p); }
-};
 
 size_t dtype_bytes(int dt) {
   switch (dt) {
@@ -183,7 +114,7 @@ __global__ __launch_bounds__(256) void ppl0_kernel(
     const int32_t *__restrict__ colPtr_r, const int32_t *__restrict__ rowIdx_r,
     const int32_t *__restrict__ packIdx_r, const float *__restrict__ val_r,
     int local_lb, int remote_lb, int L, int R, int dim, int peid) {
-  using RowT = Row<COLA_F32, EPL>;
+  using RowT = Vec<COLA_F32, EPL>;
   const int lane = threadIdx.x & 63;
   const int sub = lane / G, gl = lane % G;
   const int64_t T = (int64_t)L + R;
@@ -219,7 +150,7 @@ __global__ __launch_bounds__(256) void ppl0_kernel(
 #pragma unroll
           for (int k = 0; k < NV; ++k) {
             const int col = (k * G + gl) * EPL;
-            if (col < dim) RowT::load(rp + col, x[u][k]);
+            if (col < dim) RowT::widen(RowT::load(rp + col), x[u][k]);
             else
 #pragma unroll
               for (int q = 0; q < EPL; ++q) x[u][k][q] = 0.f;
@@ -252,7 +183,7 @@ __global__ __launch_bounds__(256) void ppl0_kernel(
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     const int col = (k * G + gl) * EPL;
-    if (col < dim) RowT::load(rp + col, acc[k]);
+    if (col < dim) RowT::widen(RowT::load(rp + col), acc[k]);
   }
   const int b = colPtr_r[idx], e = colPtr_r[idx + 1];
   for (int base = b; base < e; base += G) {
@@ -278,170 +209,47 @@ __global__ __launch_bounds__(256) void ppl0_kernel(
   }
 }
 
-// ---------------------------------------------------------------------------
-// Plan row kernel.  Each wave takes work items (one row, or one chunk of a hub
-// row); its 64/G row groups walk the item's entries with stride 64/G, U loads
-// in flight per group, then the groups' sums are combined across the wave.
-// Blocks are assigned to XCDs round-robin by the dispatcher (block b runs on
-// XCD b % 8), so block b walks XCD (b % 8)'s contiguous item range and rows
-// that share B rows share that XCD's L2.
-// ---------------------------------------------------------------------------
-template <int DT, int EPL, int G, int NV, int U>
-__global__ __launch_bounds__(256) void plan_rows_kernel(
-    const int4 *__restrict__ items, const int32_t *__restrict__ xcd_begin,
-    const int32_t *__restrict__ cols, const float *__restrict__ vals,
-    const typename Row<DT, EPL>::T *__restrict__ Bl, int64_t ldB,
-    const float *__restrict__ sl, const typename Row<DT, EPL>::T *__restrict__ Bp,
-    int64_t ldP, const float *__restrict__ sp, float *__restrict__ C, int64_t ldC,
-    float *__restrict__ partials, int pstride, int dim, int accumulate, int kahan) {
-  using RowT = Row<DT, EPL>;
-  constexpr int S = 64 / G;
-  const int lane = threadIdx.x & 63;
-  const int sub = lane / G, gl = lane % G;
-  const int wpb = blockDim.x >> 6;
-  const int wid = uniform(threadIdx.x >> 6);
-  const int xcd = blockIdx.x & 7;
-  const int nblk = (gridDim.x - xcd + 7) >> 3;
-  const int wave = (blockIdx.x >> 3) * wpb + wid;
-  const int nwave = nblk * wpb;
-  const int ib = xcd_begin[xcd], ie = xcd_begin[xcd + 1];
-
-  for (int it = ib + wave; it < ie; it += nwave) {
-    const int4 item = items[it];
-    const int b = item.y, e = item.z;
-    const bool comp = kahan && (item.w & 1);
-    float acc[NV][EPL], err[NV][EPL];
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) acc[k][q] = err[k][q] = 0.f;
-
-    for (int base = b; base < e; base += 64) {
-      const int n = min(64, e - base);
-      int mc = 0;
-      float mv = 0.f;
-      if (lane < n) {
-        mc = cols[base + lane];
-        mv = vals[base + lane];
-      }
-      for (int j = 0; j < n; j += S * U) {
-        float x[U][NV][EPL];
-        float a[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int ei = j + u * S + sub;
-          const bool ok = ei < n;
-          const int src = (ok ? ei : 0) << 2;
-          const int c = __builtin_amdgcn_ds_bpermute(src, mc);
-          float av = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv)));
-          const typename RowT::T *rp = c >= 0 ? Bl + (int64_t)c * ldB : Bp + (int64_t)(~c) * ldP;
-          if constexpr (DT == COLA_FP8_E4M3) {
-            if (ok) av *= (c >= 0 ? sl[c] : sp[~c]);
-          }
-          a[u] = ok ? av : 0.f;
-#pragma unroll
-          for (int k = 0; k < NV; ++k) {
-            const int col = (k * G + gl) * EPL;
-            if (ok && col < dim) RowT::load(rp + col, x[u][k]);
-            else
-#pragma unroll
-              for (int q = 0; q < EPL; ++q) x[u][k][q] = 0.f;
-          }
-        }
-        if (!comp) {
-#pragma unroll
-          for (int u = 0; u < U; ++u)
-#pragma unroll
-            for (int k = 0; k < NV; ++k)
-#pragma unroll
-              for (int q = 0; q < EPL; ++q) acc[k][q] = fmaf(a[u], x[u][k][q], acc[k][q]);
-        } else {
-          // Kahan: err carries the low-order part lost by acc.
-#pragma unroll
-          for (int u = 0; u < U; ++u)
-#pragma unroll
-            for (int k = 0; k < NV; ++k)
-#pragma unroll
-              for (int q = 0; q < EPL; ++q) {
-                const float y = fmaf(a[u], x[u][k][q], -err[k][q]);
-                const float t = acc[k][q] + y;
-                err[k][q] = (t - acc[k][q]) - y;
-                acc[k][q] = t;
-              }
-        }
-      }
-    }
-    // Combine the S row groups of the wave.
-#pragma unroll
-    for (int off = G; off < 64; off <<= 1)
-#pragma unroll
-      for (int k = 0; k < NV; ++k)
-#pragma unroll
-        for (int q = 0; q < EPL; ++q) {
-          if (!comp) {
-            acc[k][q] += __shfl_xor(acc[k][q], off);
-          } else {
-            // Neumaier combination of (acc, -err) pairs.
-            const float s2 = __shfl_xor(acc[k][q], off);
-            const float e2 = __shfl_xor(err[k][q], off);
-            const float s1 = acc[k][q];
-            const float t = s1 + s2;
-            const float lost = fabsf(s1) >= fabsf(s2) ? (s1 - t) + s2 : (s2 - t) + s1;
-            acc[k][q] = t;
-            err[k][q] = err[k][q] + e2 - lost;
-          }
-        }
-    if (sub == 0) {
-      float *dst;
-      bool acc_old;
-      if (item.x >= 0) {
-        dst = C + (int64_t)item.x * ldC;
-        acc_old = accumulate != 0;
-      } else {
-        dst = partials + (int64_t)(-item.x - 1) * pstride;
-        acc_old = false;
-      }
-#pragma unroll
-      for (int k = 0; k < NV; ++k) {
-        const int col = (k * G + gl) * EPL;
-        if (col < dim) {
-#pragma unroll
-          for (int q = 0; q < EPL; ++q) {
-            float v = comp ? acc[k][q] - err[k][q] : acc[k][q];
-            if (acc_old) v += dst[col + q];
-            dst[col + q] = v;
-          }
-        }
-      }
-    }
-  }
-}
-
-// One wave per hub row: partial sums of its chunks added in chunk order.
+// Hub rows: thread (h, col) adds the hub's chunk partials in chunk order.
 __global__ __launch_bounds__(256) void hub_reduce_kernel(
     const int32_t *__restrict__ hub_rows, const int32_t *__restrict__ hub_ptr, int num_hubs,
-    const float *__restrict__ partials, int pstride, float *__restrict__ C, int64_t ldC,
+    const float *__restrict__ partials, int64_t pstride, float *__restrict__ C, int64_t ldC,
     int dim, int accumulate, int kahan) {
-  const int h = (int)(((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
-  const int lane = threadIdx.x & 63;
-  if (h >= num_hubs) return;
-  const int row = hub_rows[h], c0 = hub_ptr[h], c1 = hub_ptr[h + 1];
-  float *dst = C + (int64_t)row * ldC;
-  for (int col = lane; col < dim; col += 64) {
-    float s = 0.f, err = 0.f;
-    for (int c = c0; c < c1; ++c) {
-      const float x = partials[(int64_t)c * pstride + col];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)num_hubs * dim) return;
+  const int h = (int)(t / dim), col = (int)(t % dim);
+  const int c0 = hub_ptr[h], c1 = hub_ptr[h + 1];
+  const float *p = partials + col;
+  float s = 0.f, err = 0.f;
+  int c = c0;
+  for (; c + 4 <= c1; c += 4) {
+    float x[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) x[i] = p[(int64_t)(c + i) * pstride];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
       if (kahan) {
-        const float y = x - err;
-        const float t = s + y;
-        err = (t - s) - y;
-        s = t;
+        const float y = x[i] - err;
+        const float tt = s + y;
+        err = (tt - s) - y;
+        s = tt;
       } else {
-        s += x;
+        s += x[i];
       }
     }
-    dst[col] = accumulate ? dst[col] + s : s;
   }
+  for (; c < c1; ++c) {
+    const float x = p[(int64_t)c * pstride];
+    if (kahan) {
+      const float y = x - err;
+      const float tt = s + y;
+      err = (tt - s) - y;
+      s = tt;
+    } else {
+      s += x;
+    }
+  }
+  float *dst = C + (int64_t)hub_rows[h] * ldC + col;
+  *dst = accumulate ? *dst + s : s;
 }
 
 __global__ __launch_bounds__(256) void zero_rows_kernel(const int32_t *__restrict__ rows,
@@ -522,38 +330,8 @@ __global__ __launch_bounds__(256) void error_norms_kernel(const float *__restric
 }
 
 // ---------------------------------------------------------------------------
-// Dispatch helpers.
+// Host helpers.
 // ---------------------------------------------------------------------------
-struct Shape {
-  int epl, g, nv;
-};
-
-// Lanes per row group and vectors per lane for a dim; vector form needs dim,
-// the leading dimensions and the base pointers to be multiples of 4 elements.
-bool pick_shape(int dim, bool vec_ok, Shape *s) {
-  if (dim < 1) return false;
-  if (vec_ok && dim % 4 == 0) {
-    const int chunks = dim / 4;
-    s->epl = 4;
-    if (chunks <= 8) { s->g = 8; s->nv = 1; }
-    else if (chunks <= 16) { s->g = 16; s->nv = 1; }
-    else if (chunks <= 32) { s->g = 32; s->nv = 1; }
-    else if (chunks <= 64) { s->g = 64; s->nv = 1; }
-    else if (chunks <= 128) { s->g = 64; s->nv = 2; }
-    else if (chunks <= 256) { s->g = 64; s->nv = 4; }
-    else return false;
-    return true;
-  }
-  s->epl = 1;
-  s->g = 64;
-  if (dim <= 64) s->nv = 1;
-  else if (dim <= 128) s->nv = 2;
-  else if (dim <= 256) s->nv = 4;
-  else if (dim <= 512) s->nv = 8;
-  else return false;
-  return true;
-}
-
 bool aligned(const void *p, size_t bytes) { return ((uintptr_t)p % bytes) == 0; }
 
 cola_status hip_check(hipError_t e, const char *what) {
@@ -596,15 +374,22 @@ struct cola_zip_s {
   int64_t nptr = 0, nnz = 0, n_unique = 0;
 };
 
+struct PlanBin {
+  int4 *items = nullptr;
+  int32_t *xcd = nullptr;
+  int64_t n = 0;
+};
+
 struct cola_plan_s {
   int32_t num_rows = 0, nodesPerPE = 0, U_r = 0, U_l = 0, max_dim = 0;
-  int64_t nnz = 0, num_items = 0, num_empty = 0, num_hubs = 0, num_chunks = 0;
-  int4 *items = nullptr;
-  int32_t *xcd_begin = nullptr, *cols = nullptr, *empty_rows = nullptr,
-          *hub_rows = nullptr, *hub_ptr = nullptr, *remote_cols = nullptr;
+  int64_t nnz = 0, num_empty = 0, num_hubs = 0, num_chunks = 0;
+  PlanBin shortb, longb;
+  int32_t *cols = nullptr, *empty_rows = nullptr, *hub_rows = nullptr, *hub_ptr = nullptr,
+          *remote_cols = nullptr;
   float *vals = nullptr, *partials = nullptr;
+  int64_t pstride = 0;
   int64_t device_bytes = 0;
-  int block_threads = 256, blocks_per_cu = 8;
+  cola_plan_tuning tune{0, 0, 1, 8, 1};
 };
 
 static void free_zip(cola_zip_s *z) {
@@ -620,8 +405,10 @@ static void free_zip(cola_zip_s *z) {
 
 static void free_plan(cola_plan_s *p) {
   if (!p) return;
-  (void)hipFree(p->items);
-  (void)hipFree(p->xcd_begin);
+  for (PlanBin *b : {&p->shortb, &p->longb}) {
+    (void)hipFree(b->items);
+    (void)hipFree(b->xcd);
+  }
   (void)hipFree(p->cols);
   (void)hipFree(p->vals);
   (void)hipFree(p->empty_rows);
@@ -648,49 +435,113 @@ void launch_ppl0(float *out, const float *in, const float *packed, const int32_t
                      in, packed, rp, ci, ri, vl, cp, rr, pi, vr, llb, rlb, L, R, dim, peid);
 }
 
-template <int DT, int EPL, int G, int NV>
-void launch_plan_rows(const cola_plan_s *p, float *C, int64_t ldC, const void *Bl, int64_t ldB,
-                      const float *sl, const void *Bp, int64_t ldP, const float *sp, int dim,
-                      int accumulate, int kahan, hipStream_t s) {
-  using T = typename Row<DT, EPL>::T;
-  // Loads in flight per row group: about 16 entries per wave per batch,
-  // fewer when each entry already spans several vectors per lane.
-  constexpr int S = 64 / G;
-  constexpr int U0 = (16 / S) < 1 ? 1 : (16 / S);
-  constexpr int U1 = U0 > 8 ? 8 : U0;
-  constexpr int U = NV > 1 ? ((U1 / NV) < 1 ? 1 : U1 / NV) : U1;
-  const int wpb = p->block_threads / 64;
-  int64_t blocks = (p->num_items + wpb - 1) / wpb;
-  blocks = std::min<int64_t>(blocks, (int64_t)device_cu_count() * p->blocks_per_cu);
-  blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
-  hipLaunchKernelGGL((plan_rows_kernel<DT, EPL, G, NV, U>), dim3((unsigned)blocks),
-                     dim3(p->block_threads), 0, s, p->items, p->xcd_begin, p->cols, p->vals,
-                     (const T *)Bl, ldB, sl, (const T *)Bp, ldP, sp, C, ldC, p->partials,
-                     p->max_dim, dim, accumulate, kahan);
+int vec_elems(int dt) { return dt == COLA_F32 ? 4 : (dt == COLA_FP8_E4M3 ? 16 : 8); }
+
+bool launch_rows(int dt, const Shape &sh, const cola_dev::RowArgs &a, int blocks, hipStream_t s) {
+  switch (dt) {
+    case COLA_F32: return cola_dev::launch_rows_f32(sh, a, blocks, 256, s);
+    case COLA_BF16: return cola_dev::launch_rows_bf16(sh, a, blocks, 256, s);
+    case COLA_F16: return cola_dev::launch_rows_f16(sh, a, blocks, 256, s);
+    case COLA_FP8_E4M3: return cola_dev::launch_rows_fp8(sh, a, blocks, 256, s);
+  }
+  return false;
 }
 
-template <int DT>
-bool dispatch_plan(const Shape &sh, const cola_plan_s *p, float *C, int64_t ldC, const void *Bl,
-                   int64_t ldB, const float *sl, const void *Bp, int64_t ldP, const float *sp,
-                   int dim, int accumulate, int kahan, hipStream_t s) {
-#define COLA_PLAN_CASE(E, G_, NV_)                                                          \
-  if (sh.epl == E && sh.g == G_ && sh.nv == NV_) {                                          \
-    launch_plan_rows<DT, E, G_, NV_>(p, C, ldC, Bl, ldB, sl, Bp, ldP, sp, dim, accumulate, \
-                                     kahan, s);                                             \
-    return true;                                                                            \
+int clamp_rpw(int req, int S) {
+  int r = 1;
+  while (r * 2 <= req && r * 2 <= S) r *= 2;
+  return r;
+}
+
+struct SpmmCall {
+  float *C;
+  int64_t ldC;
+  const void *Bl;
+  int64_t ldB;
+  const float *sl;
+  const void *Bp;
+  int64_t ldP;
+  const float *sp;
+  int dim;
+  int dtype;
+  int flags;
+};
+
+// Validates a call and runs it with the given tuning.
+cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tuning &t,
+                     hipStream_t s) {
+  const size_t eb = dtype_bytes(c.dtype);
+  if (!eb) return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: unknown dtype");
+  if (c.flags & ~(COLA_OVERWRITE | COLA_KAHAN)) return fail(COLA_ERR_INVALID, "spmm_partition: unknown flag");
+  if (c.dim < 1 || c.ldC < c.dim || c.ldB < c.dim || (p->U_r > 0 && c.ldP < c.dim))
+    return fail(COLA_ERR_INVALID, "spmm_partition: bad dim or leading dimension");
+  if (c.dim > p->max_dim && p->num_chunks > 0)
+    return fail(COLA_ERR_INVALID, "spmm_partition: dim exceeds the plan's max_dim");
+  if (p->num_rows > 0 && !c.C) return fail(COLA_ERR_INVALID, "spmm_partition: C is NULL");
+  if (p->U_l > 0 && !c.Bl) return fail(COLA_ERR_INVALID, "spmm_partition: B_local is NULL");
+  if (p->U_r > 0 && !c.Bp) return fail(COLA_ERR_INVALID, "spmm_partition: packed is NULL");
+  if (c.dtype == COLA_FP8_E4M3 && ((p->U_l > 0 && !c.sl) || (p->U_r > 0 && !c.sp)))
+    return fail(COLA_ERR_INVALID, "spmm_partition: FP8 needs row scales");
+  const int ev = vec_elems(c.dtype);
+  const bool vec_ok = c.ldC % 4 == 0 && c.ldB % ev == 0 && (p->U_r == 0 || c.ldP % ev == 0) &&
+                      aligned(c.C, 16) && aligned(c.Bl, 16) && (p->U_r == 0 || aligned(c.Bp, 16));
+  // Column tile: a multiple of the vector width (so tiles stay aligned).
+  int tile = c.dim;
+  if (t.col_tile > 0 && t.col_tile < c.dim && vec_ok && c.dim % ev == 0) {
+    tile = std::max(ev, t.col_tile / ev * ev);
   }
-  COLA_PLAN_CASE(4, 8, 1)
-  COLA_PLAN_CASE(4, 16, 1)
-  COLA_PLAN_CASE(4, 32, 1)
-  COLA_PLAN_CASE(4, 64, 1)
-  COLA_PLAN_CASE(4, 64, 2)
-  COLA_PLAN_CASE(4, 64, 4)
-  COLA_PLAN_CASE(1, 64, 1)
-  COLA_PLAN_CASE(1, 64, 2)
-  COLA_PLAN_CASE(1, 64, 4)
-  COLA_PLAN_CASE(1, 64, 8)
-#undef COLA_PLAN_CASE
-  return false;
+  const int accumulate = (c.flags & COLA_OVERWRITE) ? 0 : 1;
+  const int kahan = (c.flags & COLA_KAHAN) ? 1 : 0;
+  if (!accumulate && p->num_empty > 0) {
+    const int64_t blocks = (p->num_empty * 64 + 255) / 256;
+    hipLaunchKernelGGL(zero_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p->empty_rows,
+                       (int)p->num_empty, c.C, c.ldC, c.dim);
+  }
+  const int bpc = t.blocks_per_cu > 0 ? t.blocks_per_cu : 8;
+  for (int c0 = 0; c0 < c.dim; c0 += tile) {
+    const int w = std::min(tile, c.dim - c0);
+    Shape sh;
+    if (!cola_dev::pick_shape(c.dtype, w, vec_ok, &sh))
+      return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: dim " + std::to_string(c.dim) + " not supported");
+    const int S = 64 / sh.g;
+    for (const PlanBin *bin : {&p->longb, &p->shortb}) {
+      if (bin->n == 0) continue;
+      cola_dev::RowArgs a;
+      a.items = bin->items;
+      a.xcd_begin = bin->xcd;
+      a.cols = p->cols;
+      a.vals = p->vals;
+      a.Bl = c.Bl ? (const char *)c.Bl + (size_t)c0 * eb : nullptr;
+      a.ldB = c.ldB;
+      a.sl = c.sl;
+      a.Bp = c.Bp ? (const char *)c.Bp + (size_t)c0 * eb : nullptr;
+      a.ldP = c.ldP;
+      a.sp = c.sp;
+      a.C = c.C + c0;
+      a.ldC = c.ldC;
+      a.partials = p->partials ? p->partials + c0 : nullptr;
+      a.pstride = p->pstride;
+      a.dim = w;
+      a.accumulate = accumulate;
+      a.kahan = kahan;
+      const int req = bin == &p->shortb ? (t.rpw_short > 0 ? t.rpw_short : S) : (t.rpw_long > 0 ? t.rpw_long : 1);
+      a.rpw = clamp_rpw(req, S);
+      a.xcd_remap = t.xcd_remap ? 1 : 0;
+      const int64_t waves = (bin->n + a.rpw - 1) / a.rpw;
+      int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bpc);
+      blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
+      if (!launch_rows(c.dtype, sh, a, (int)blocks, s))
+        return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: no kernel for this dim");
+    }
+  }
+  if (p->num_hubs > 0) {
+    const int64_t threads = p->num_hubs * (int64_t)c.dim;
+    const int64_t blocks = (threads + 255) / 256;
+    hipLaunchKernelGGL(hub_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p->hub_rows,
+                       p->hub_ptr, (int)p->num_hubs, p->partials, p->pstride, c.C, c.ldC, c.dim,
+                       accumulate, kahan);
+  }
+  return hip_check(hipGetLastError(), "spmm_partition launch");
 }
 
 }  // namespace
@@ -827,7 +678,7 @@ cola_status cola_ppl0_batch_spmm(float *output, const float *input, const float 
   const bool vec_ok = aligned(output, 16) && (!input || aligned(input, 16)) &&
                       (!packed || aligned(packed, 16));
   Shape sh;
-  if (!pick_shape(dim, vec_ok, &sh))
+  if (!cola_dev::pick_shape(COLA_F32, dim, vec_ok, &sh))
     return fail(COLA_ERR_UNSUPPORTED, "ppl0_batch_spmm: dim " + std::to_string(dim) + " not supported");
   hipStream_t s = (hipStream_t)stream;
 #define COLA_PPL0_CASE(E, G_, NV_)                                                             \
@@ -838,6 +689,8 @@ cola_status cola_ppl0_batch_spmm(float *output, const float *input, const float 
                             peid, s);                                                           \
     return hip_check(hipGetLastError(), "ppl0_batch_spmm launch");                             \
   }
+  COLA_PPL0_CASE(4, 2, 1)
+  COLA_PPL0_CASE(4, 4, 1)
   COLA_PPL0_CASE(4, 8, 1)
   COLA_PPL0_CASE(4, 16, 1)
   COLA_PPL0_CASE(4, 32, 1)
@@ -875,7 +728,7 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
   *out = nullptr;
   cola_plan_opts o{};
   if (opts) o = *opts;
-  if (o.hub_threshold < 0 || o.chunk_nnz < 0 || o.kahan_degree < 0 || o.max_dim < 0)
+  if (o.hub_threshold < 0 || o.chunk_nnz < 0 || o.kahan_degree < 0 || o.max_dim < 0 || o.short_degree < 0)
     return fail(COLA_ERR_INVALID, "plan_create: negative option");
   cola::HostPlan hp;
   cola_status st = cola::build_plan(part->p, o, &hp);
@@ -887,15 +740,21 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
   p->U_r = hp.U_r;
   p->U_l = hp.U_l;
   p->max_dim = hp.max_dim;
+  p->pstride = (hp.max_dim + 15) / 16 * 16;
   p->nnz = hp.nnz;
-  p->num_items = (int64_t)hp.items.size();
   p->num_empty = (int64_t)hp.empty_rows.size();
   p->num_hubs = (int64_t)hp.hub_rows.size();
   p->num_chunks = hp.hub_chunk_ptr.back();
-  std::vector<int4> items(hp.items.size());
-  for (size_t i = 0; i < items.size(); ++i)
-    items[i] = make_int4(hp.items[i].out, hp.items[i].begin, hp.items[i].end, hp.items[i].flags);
-  if ((st = upload(items, &p->items)) || (st = upload(hp.xcd_begin, &p->xcd_begin)) ||
+  auto to_int4 = [](const std::vector<cola::WorkItem> &v) {
+    std::vector<int4> r(v.size());
+    for (size_t i = 0; i < v.size(); ++i) r[i] = make_int4(v[i].out, v[i].begin, v[i].end, v[i].flags);
+    return r;
+  };
+  std::vector<int4> is = to_int4(hp.items_short), il = to_int4(hp.items_long);
+  p->shortb.n = (int64_t)is.size();
+  p->longb.n = (int64_t)il.size();
+  if ((st = upload(is, &p->shortb.items)) || (st = upload(hp.xcd_short, &p->shortb.xcd)) ||
+      (st = upload(il, &p->longb.items)) || (st = upload(hp.xcd_long, &p->longb.xcd)) ||
       (st = upload(hp.cols, &p->cols)) || (st = upload(hp.vals, &p->vals)) ||
       (st = upload(hp.empty_rows, &p->empty_rows)) || (st = upload(hp.hub_rows, &p->hub_rows)) ||
       (st = upload(hp.hub_chunk_ptr, &p->hub_ptr)) || (st = upload(hp.remote_cols, &p->remote_cols))) {
@@ -903,17 +762,17 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
     return st;
   }
   if (p->num_chunks > 0) {
-    st = hip_check(hipMalloc((void **)&p->partials, (size_t)p->num_chunks * p->max_dim * sizeof(float)),
+    st = hip_check(hipMalloc((void **)&p->partials, (size_t)p->num_chunks * p->pstride * sizeof(float)),
                    "hipMalloc partials");
     if (st) {
       free_plan(p);
       return st;
     }
   }
-  p->device_bytes = (int64_t)(items.size() * sizeof(int4) + hp.xcd_begin.size() * 4 + hp.cols.size() * 4 +
+  p->device_bytes = (int64_t)((is.size() + il.size()) * sizeof(int4) + 18 * 4 + hp.cols.size() * 4 +
                               hp.vals.size() * 4 + hp.empty_rows.size() * 4 + hp.hub_rows.size() * 4 +
                               hp.hub_chunk_ptr.size() * 4 + hp.remote_cols.size() * 4 +
-                              (size_t)p->num_chunks * p->max_dim * 4);
+                              (size_t)p->num_chunks * p->pstride * 4);
   *out = p;
   return COLA_OK;
 }
@@ -921,13 +780,30 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
 cola_status cola_plan_get_info(cola_plan p, cola_plan_info *info) {
   if (!p || !info) return fail(COLA_ERR_INVALID, "plan_info: NULL argument");
   info->num_rows = p->num_rows;
-  info->num_items = p->num_items;
+  info->num_items = p->shortb.n + p->longb.n;
   info->num_hub_rows = p->num_hubs;
   info->num_hub_chunks = p->num_chunks;
   info->nnz = p->nnz;
   info->U_r = p->U_r;
   info->U_l = p->U_l;
   info->device_bytes = p->device_bytes;
+  info->num_short_items = p->shortb.n;
+  info->num_long_items = p->longb.n;
+  return COLA_OK;
+}
+
+cola_status cola_plan_set_tuning(cola_plan p, const cola_plan_tuning *t) {
+  if (!p || !t) return fail(COLA_ERR_INVALID, "plan_set_tuning: NULL argument");
+  if (t->col_tile < 0 || t->rpw_short < 0 || t->rpw_short > 64 || t->rpw_long < 0 || t->rpw_long > 64 ||
+      t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || (t->xcd_remap != 0 && t->xcd_remap != 1))
+    return fail(COLA_ERR_INVALID, "plan_set_tuning: parameter out of range");
+  p->tune = *t;
+  return COLA_OK;
+}
+
+cola_status cola_plan_get_tuning(cola_plan p, cola_plan_tuning *t) {
+  if (!p || !t) return fail(COLA_ERR_INVALID, "plan_get_tuning: NULL argument");
+  *t = p->tune;
   return COLA_OK;
 }
 
@@ -947,61 +823,96 @@ cola_status cola_spmm_partition(cola_plan p, float *C, int64_t ld_C, const void 
                                 int64_t ld_packed, const float *packed_scales, int dim,
                                 cola_dtype dtype, int flags, void *stream) {
   if (!p) return fail(COLA_ERR_INVALID, "spmm_partition: NULL plan");
-  const size_t eb = dtype_bytes(dtype);
-  if (!eb) return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: unknown dtype");
-  if (flags & ~(COLA_OVERWRITE | COLA_KAHAN)) return fail(COLA_ERR_INVALID, "spmm_partition: unknown flag");
-  if (dim < 1 || ld_C < dim || ld_B < dim || (p->U_r > 0 && ld_packed < dim))
-    return fail(COLA_ERR_INVALID, "spmm_partition: bad dim or leading dimension");
-  if (dim > p->max_dim && p->num_chunks > 0)
-    return fail(COLA_ERR_INVALID, "spmm_partition: dim exceeds the plan's max_dim");
-  if (p->num_rows > 0 && !C) return fail(COLA_ERR_INVALID, "spmm_partition: C is NULL");
-  if (p->nnz > 0 && !B_local && p->U_l > 0) return fail(COLA_ERR_INVALID, "spmm_partition: B_local is NULL");
-  if (p->U_r > 0 && !packed) return fail(COLA_ERR_INVALID, "spmm_partition: packed is NULL");
-  if (dtype == COLA_FP8_E4M3 && ((p->U_l > 0 && !scales_local) || (p->U_r > 0 && !packed_scales)))
-    return fail(COLA_ERR_INVALID, "spmm_partition: FP8 needs row scales");
-  const bool vec_ok = ld_C % 4 == 0 && ld_B % 4 == 0 && (p->U_r == 0 || ld_packed % 4 == 0) &&
-                      aligned(C, 16) && aligned(B_local, 4 * eb) &&
-                      (p->U_r == 0 || aligned(packed, 4 * eb));
-  Shape sh;
-  if (!pick_shape(dim, vec_ok, &sh))
-    return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: dim " + std::to_string(dim) + " not supported");
+  SpmmCall c{C, ld_C, B_local, ld_B, scales_local, packed, ld_packed, packed_scales, dim, (int)dtype, flags};
+  return run_plan(p, c, p->tune, (hipStream_t)stream);
+}
+
+cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *B_local,
+                               int64_t ld_B, const float *scales_local, const void *packed,
+                               int64_t ld_packed, const float *packed_scales, int dim,
+                               cola_dtype dtype, int flags, int iters, void *stream,
+                               cola_plan_tuning *best_out, float *best_ms) {
+  if (!p) return fail(COLA_ERR_INVALID, "plan_autotune: NULL plan");
+  if (iters < 1) iters = 3;
   hipStream_t s = (hipStream_t)stream;
-  const int accumulate = (flags & COLA_OVERWRITE) ? 0 : 1;
-  const int kahan = (flags & COLA_KAHAN) ? 1 : 0;
-  if (!accumulate && p->num_empty > 0) {
-    const int64_t blocks = (p->num_empty * 64 + 255) / 256;
-    hipLaunchKernelGGL(zero_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p->empty_rows,
-                       (int)p->num_empty, C, ld_C, dim);
+  SpmmCall c{C, ld_C, B_local, ld_B, scales_local, packed, ld_packed, packed_scales, dim, (int)dtype,
+             COLA_OVERWRITE | (flags & COLA_KAHAN)};
+  hipEvent_t e0, e1;
+  cola_status st;
+  if ((st = hip_check(hipEventCreate(&e0), "hipEventCreate"))) return st;
+  if ((st = hip_check(hipEventCreate(&e1), "hipEventCreate"))) {
+    (void)hipEventDestroy(e0);
+    return st;
   }
-  if (p->num_items > 0) {
-    bool ok = false;
-    switch (dtype) {
-      case COLA_F32:
-        ok = dispatch_plan<COLA_F32>(sh, p, C, ld_C, B_local, ld_B, scales_local, packed, ld_packed,
-                                     packed_scales, dim, accumulate, kahan, s);
-        break;
-      case COLA_BF16:
-        ok = dispatch_plan<COLA_BF16>(sh, p, C, ld_C, B_local, ld_B, scales_local, packed, ld_packed,
-                                      packed_scales, dim, accumulate, kahan, s);
-        break;
-      case COLA_F16:
-        ok = dispatch_plan<COLA_F16>(sh, p, C, ld_C, B_local, ld_B, scales_local, packed, ld_packed,
-                                     packed_scales, dim, accumulate, kahan, s);
-        break;
-      case COLA_FP8_E4M3:
-        ok = dispatch_plan<COLA_FP8_E4M3>(sh, p, C, ld_C, B_local, ld_B, scales_local, packed,
-                                          ld_packed, packed_scales, dim, accumulate, kahan, s);
-        break;
+  auto time_of = [&](const cola_plan_tuning &t, float *ms) -> cola_status {
+    cola_status r = run_plan(p, c, t, s);  // warm-up (and validation)
+    if (r) return r;
+    if ((r = hip_check(hipEventRecord(e0, s), "hipEventRecord"))) return r;
+    for (int i = 0; i < iters; ++i)
+      if ((r = run_plan(p, c, t, s))) return r;
+    if ((r = hip_check(hipEventRecord(e1, s), "hipEventRecord"))) return r;
+    if ((r = hip_check(hipEventSynchronize(e1), "hipEventSynchronize"))) return r;
+    float t_ms = 0.f;
+    if ((r = hip_check(hipEventElapsedTime(&t_ms, e0, e1), "hipEventElapsedTime"))) return r;
+    *ms = t_ms / iters;
+    return COLA_OK;
+  };
+  const char *env = getenv("COLA_AUTOTUNE_VERBOSE");
+  const bool verbose = env && env[0] == '1';
+  cola_plan_tuning best = p->tune;
+  float bms = 0.f;
+  st = time_of(best, &bms);
+  const int ev = vec_elems(dtype);
+  // Coordinate descent over one knob at a time.
+  for (int pass = 0; pass < 2 && !st; ++pass) {
+    std::vector<cola_plan_tuning> cands;
+    for (int tile : {0, dim / 2, dim / 4, dim / 8}) {
+      if (tile != 0 && (tile < ev || tile % ev != 0)) continue;
+      cola_plan_tuning t = best;
+      t.col_tile = tile;
+      cands.push_back(t);
     }
-    if (!ok) return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: no kernel for this dim");
+    for (int r : {1, 2, 4, 8, 16, 32, 0}) {
+      cola_plan_tuning t = best;
+      t.rpw_short = r;
+      cands.push_back(t);
+    }
+    for (int r : {1, 2, 4}) {
+      cola_plan_tuning t = best;
+      t.rpw_long = r;
+      cands.push_back(t);
+    }
+    for (int b : {2, 4, 8, 16}) {
+      cola_plan_tuning t = best;
+      t.blocks_per_cu = b;
+      cands.push_back(t);
+    }
+    for (int x : {0, 1}) {
+      cola_plan_tuning t = best;
+      t.xcd_remap = x;
+      cands.push_back(t);
+    }
+    for (const auto &t : cands) {
+      float ms = 0.f;
+      if ((st = time_of(t, &ms))) break;
+      if (verbose)
+        fprintf(stderr, "[autotune] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d: %.4f ms\n",
+                t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.xcd_remap, ms);
+      if (ms < bms * 0.99f) {
+        bms = ms;
+        best = t;
+      }
+    }
   }
-  if (p->num_hubs > 0) {
-    const int64_t blocks = (p->num_hubs * 64 + 255) / 256;
-    hipLaunchKernelGGL(hub_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p->hub_rows,
-                       p->hub_ptr, (int)p->num_hubs, p->partials, p->max_dim, C, ld_C, dim,
-                       accumulate, kahan);
-  }
-  return hip_check(hipGetLastError(), "spmm_partition launch");
+  if (!st) st = run_plan(p, c, best, s);
+  if (!st) st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) return st;
+  p->tune = best;
+  if (best_out) *best_out = best;
+  if (best_ms) *best_ms = bms;
+  return COLA_OK;
 }
 
 cola_status cola_quantize_rows(const float *src, int64_t ld_src, int64_t rows, int dim,

@@ -142,8 +142,8 @@ def dist_spmm_ppl_batch(output, B_global, packed, csr_zip_l, csc_zip_r, dim, nod
 class Plan:
     """Degree-binned, atomic-free plan of one partition (device resident)."""
 
-    def __init__(self, partition, hub_threshold=0, chunk_nnz=0, kahan_degree=0, max_dim=0):
-        o = _lib.PlanOpts(int(hub_threshold), int(chunk_nnz), int(kahan_degree), int(max_dim))
+    def __init__(self, partition, hub_threshold=0, chunk_nnz=0, kahan_degree=0, max_dim=0, short_degree=0):
+        o = _lib.PlanOpts(int(hub_threshold), int(chunk_nnz), int(kahan_degree), int(max_dim), int(short_degree))
         h = ctypes.c_void_p()
         check(lib().cola_plan_create(partition.handle, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
@@ -178,6 +178,35 @@ class Plan:
         check(lib().cola_spmm_partition(self._h, _dptr(C), C.stride(0), _dptr(B_local), B_local.stride(0),
                                         _dptr(scales_local), _dptr(packed), ldp, _dptr(packed_scales),
                                         int(dim), int(dtype), int(flags), _stream(stream)))
+
+    TUNING_FIELDS = [f for f, _ in _lib.PlanTuning._fields_]
+
+    @property
+    def tuning(self):
+        t = _lib.PlanTuning()
+        check(lib().cola_plan_get_tuning(self._h, ctypes.byref(t)))
+        return {f: getattr(t, f) for f in self.TUNING_FIELDS}
+
+    @tuning.setter
+    def tuning(self, d):
+        cur = self.tuning
+        cur.update(d)
+        t = _lib.PlanTuning(*[int(cur[f]) for f in self.TUNING_FIELDS])
+        check(lib().cola_plan_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, C, B_local, packed, dim=None, dtype=F32, flags=OVERWRITE, scales_local=None,
+                 packed_scales=None, iters=3, stream=None):
+        """Times candidate launch parameters on these buffers and keeps the
+        fastest; C is overwritten with A_p·B.  Returns (tuning dict, ms)."""
+        dim = dim if dim is not None else C.shape[1]
+        ldp = packed.stride(0) if packed is not None and packed.dim() == 2 else dim
+        best = _lib.PlanTuning()
+        ms = ctypes.c_float()
+        check(lib().cola_plan_autotune(self._h, _dptr(C), C.stride(0), _dptr(B_local), B_local.stride(0),
+                                       _dptr(scales_local), _dptr(packed), ldp, _dptr(packed_scales), int(dim),
+                                       int(dtype), int(flags), int(iters), _stream(stream), ctypes.byref(best),
+                                       ctypes.byref(ms)))
+        return {f: getattr(best, f) for f in self.TUNING_FIELDS}, ms.value
 
 
 def quantize_rows(src, dtype, dst=None, scales=None, stream=None):

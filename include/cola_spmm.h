@@ -239,14 +239,44 @@ typedef struct {
                             COLA_KAHAN is passed (0 = default 1024)         */
   int32_t max_dim;       /* largest dim the hub workspace is sized for
                             (0 = default 256)                               */
+  int32_t short_degree;  /* rows with at most this many entries form the
+                            short-row bin (0 = default 32)                  */
 } cola_plan_opts;
 cola_status cola_plan_create(cola_partition p, const cola_plan_opts *opts,
                              cola_plan *out);
 typedef struct {
   int64_t num_rows, num_items, num_hub_rows, num_hub_chunks, nnz, U_r, U_l;
   int64_t device_bytes;
+  int64_t num_short_items, num_long_items;
 } cola_plan_info;
 cola_status cola_plan_get_info(cola_plan plan, cola_plan_info *info);
+
+/* Launch parameters of the plan's kernels (the autotuner's search space). */
+typedef struct {
+  int32_t col_tile;      /* dense columns per pass over B (0 = all of dim);
+                            rounded down to the 16-byte vector width        */
+  int32_t rpw_short;     /* short-row bin: rows in flight per wave (power of
+                            two; 0 = one row per lane group)                */
+  int32_t rpw_long;      /* long-row bin: rows in flight per wave (0 = 1)   */
+  int32_t blocks_per_cu; /* grid size in 256-thread blocks per CU (0 = 8)   */
+  int32_t xcd_remap;     /* 1: each XCD walks a contiguous row range
+                            (default); 0: plain block order                 */
+} cola_plan_tuning;
+cola_status cola_plan_set_tuning(cola_plan plan, const cola_plan_tuning *t);
+cola_status cola_plan_get_tuning(cola_plan plan, cola_plan_tuning *t);
+
+/* Project-written autotuner: times candidate launch parameters (column tile,
+ * rows per wave of each bin, grid size, XCD remap) by coordinate descent on
+ * the caller's buffers with HIP events, keeps the fastest in the plan and
+ * returns it (and its time).  C is used as scratch: on return it holds
+ * A_p·B (overwrite).  Blocks the calling thread. */
+cola_status cola_plan_autotune(cola_plan plan, float *C, int64_t ld_C,
+                               const void *B_local, int64_t ld_B,
+                               const float *scales_local, const void *packed,
+                               int64_t ld_packed, const float *packed_scales,
+                               int dim, cola_dtype dtype, int flags, int iters,
+                               void *stream, cola_plan_tuning *best,
+                               float *best_ms);
 /* Device array of the unique remote column ids (U_r entries), for packing. */
 cola_status cola_plan_remote_cols(cola_plan plan, const int32_t **d_cols);
 cola_status cola_plan_destroy(cola_plan plan);

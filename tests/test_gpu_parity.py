@@ -359,3 +359,49 @@ def test_full_size_partition_checksum():
     Ch = C.cpu().numpy()
     refs = np.stack([orc.spmm_f64(rp, ci, va, int(i), int(i) + 1, Bh[:g.n])[0] for i in sample])
     assert rel_fro(Ch[sample], refs) <= FP32_TOL
+
+
+TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1),
+           dict(col_tile=32, rpw_short=1, rpw_long=2, blocks_per_cu=2, xcd_remap=0),
+           dict(col_tile=16, rpw_short=2, rpw_long=4, blocks_per_cu=16, xcd_remap=1),
+           dict(col_tile=48, rpw_short=64, rpw_long=64, blocks_per_cu=1, xcd_remap=0)]
+
+
+@pytest.mark.parametrize("tuning", TUNINGS)
+@pytest.mark.parametrize("dtype", [cola.F32, cola.BF16, cola.FP8_E4M3])
+def test_tunings_give_the_same_result(c1, tuning, dtype):
+    """Column tiles, rows per wave, grid size and XCD remap change the schedule,
+    never the result: the same FP32-accumulated sums up to rounding order."""
+    dim = 64
+    plan = cola.Plan(c1.part, short_degree=24, hub_threshold=256, chunk_nnz=128)
+    B = c1.B
+    Bq, sc = cola.quantize_rows(B, dtype)
+    tdt = {cola.F32: torch.float32, cola.BF16: torch.uint16, cola.FP8_E4M3: torch.uint8}[dtype]
+    packed = torch.empty((plan.U_r, dim), dtype=tdt, device=B.device)
+    psc = torch.empty(plan.U_r, dtype=torch.float32, device=B.device) if dtype == cola.FP8_E4M3 else None
+    plan.pack(Bq, packed, dtype=dtype, scales_global=sc, packed_scales=psc)
+    outs = []
+    for t in (TUNINGS[0], tuning):
+        plan.tuning = t
+        C = torch.full((c1.P, dim), 3.0, dtype=torch.float32, device=B.device)
+        plan.spmm(C, Bq[:c1.P], packed, dim=dim, dtype=dtype, flags=cola.OVERWRITE,
+                  scales_local=None if sc is None else sc[:c1.P], packed_scales=psc)
+        outs.append(C.cpu().numpy())
+    torch.cuda.synchronize()
+    assert rel_fro(outs[1], outs[0].astype(np.float64)) <= 1e-6
+    if dtype == cola.F32:
+        assert rel_fro(outs[1][:c1.M], c1.ref) <= FP32_TOL
+
+
+def test_autotune_keeps_a_correct_schedule(c1):
+    plan = cola.Plan(c1.part)
+    packed = torch.empty((plan.U_r, c1.dim), dtype=torch.float32, device=c1.dev)
+    plan.pack(c1.B, packed)
+    C = torch.zeros((c1.P, c1.dim), dtype=torch.float32, device=c1.dev)
+    best, ms = plan.autotune(C, c1.B_local(), packed, iters=2)
+    assert ms > 0 and plan.tuning == best
+    assert rel_fro(C.cpu().numpy()[:c1.M], c1.ref) <= FP32_TOL
+    C2 = torch.zeros_like(C)
+    plan.spmm(C2, c1.B_local(), packed, flags=cola.OVERWRITE)
+    torch.cuda.synchronize()
+    assert rel_fro(C2.cpu().numpy()[:c1.M], c1.ref) <= FP32_TOL
