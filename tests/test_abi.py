@@ -17,7 +17,8 @@ def declared_functions():
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         text = re.sub(r"//[^\n]*", "", text)
-        names |= set(re.findall(r"\b(cola_[a-z0-9_]+)\s*\(", text))
+        # C-ABI declarations: a cola_status / const char* return type at line start
+        names |= set(re.findall(r"^(?:cola_status|const char \*)\s*(cola_[a-z0-9_]+)\s*\(", text, flags=re.M))
     return names
 
 
@@ -47,3 +48,12 @@ def test_version_and_error_text():
     st = L.cola_rmat_graph(0, 1.0, 0.5, 0.2, 0.2, 1, 0, 1, ctypes.byref(h))
     assert st == _lib.COLA_ERR_INVALID
     assert b"log_n" in L.cola_last_error()
+
+
+def test_cpp_drop_in_driver_compiles_and_links(tmp_path):
+    """Reference-style host code (vector-taking d_csr_zip / d_csc_zip
+    constructors, nnz_split, single-stage wrapper) builds against
+    include/sparse_data.h and the library."""
+    from tests.cpp_driver import build_driver
+    exe = build_driver(tmp_path)
+    assert os.path.exists(exe)

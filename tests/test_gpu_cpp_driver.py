@@ -1,0 +1,26 @@
+"""Runs the C++ drop-in driver (tests/cpp/drop_in_driver.cpp) on partition
+files written by cola_serialize: the reference's own known-answer check,
+B = 1 and values = 1 => C[i,:] = deg(i), passes for every rank."""
+import subprocess
+
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import cola_amd as cola  # noqa: E402
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a GPU")]
+
+
+@pytest.mark.parametrize("dim", [32, 128, 96])
+def test_cpp_driver_known_answer(tmp_path, dim):
+    from tests.cpp_driver import build_driver
+    exe = build_driver(tmp_path)
+    g = cola.Graph.rmat(13, 12, seed=21, reorder=True, gcn=False)
+    cola.serialize(g, str(tmp_path), "rmat13", [4], write_values=False)
+    for rank in range(4):
+        out = subprocess.run([exe, str(tmp_path), "rmat13", "4", str(rank), str(dim)], capture_output=True,
+                             text=True, timeout=60)
+        assert out.returncode == 0, out.stdout + out.stderr
+        assert f"PEid-{rank} passed" in out.stdout
