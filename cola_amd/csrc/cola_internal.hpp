@@ -38,7 +38,8 @@ struct WorkItem {  // 16 bytes, loaded as one int4 by the row kernel
   int32_t out;     // >= 0: output row; < 0: -(hub chunk + 1)
   int32_t begin;   // first entry in the merged entry list
   int32_t end;     // one past the last entry
-  int32_t flags;   // bit 0: compensated (Kahan) summation allowed
+  int32_t flags;   // bit 0: compensated (Kahan) summation allowed;
+                   // bits 1..: number of local entries (they come first)
 };
 
 struct HostPlan {
@@ -54,6 +55,10 @@ struct HostPlan {
   // range per XCD (9 entries).
   std::vector<WorkItem> items_short, items_long;
   std::vector<int32_t> xcd_short, xcd_long;
+  // The same items restricted to those with remote entries (second phase of
+  // the local-then-remote schedule).
+  std::vector<WorkItem> items_short_r, items_long_r;
+  std::vector<int32_t> xcd_short_r, xcd_long_r;
   std::vector<int32_t> empty_rows;   // rows without entries (OVERWRITE)
   std::vector<int32_t> hub_rows;     // output row of each hub
   std::vector<int32_t> hub_chunk_ptr;// chunk range per hub (num_hubs+1)

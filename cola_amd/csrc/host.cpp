@@ -449,6 +449,11 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
       out->vals[cur[r]++] = p.csc_val_r[k];
     }
 
+  // Local entries of each row come first: lcnt[i] of them.
+  std::vector<int64_t> lcnt(M, 0);
+  for (size_t z = 0; z < p.csr_rowIdx_l.size(); ++z)
+    lcnt[p.csr_rowIdx_l[z]] += p.csr_rowPtr_l[z + 1] - p.csr_rowPtr_l[z];
+
   // Distinct local columns (for the algorithmic byte count).
   {
     std::vector<uint8_t> used(p.nodesPerPE, 0);
@@ -458,29 +463,40 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
     out->U_l = (int32_t)ul;
   }
 
-  // Work items in row order; hub rows become chunk items.
+  // Work items in row order; hub rows become chunk items.  flags: bit 0 =
+  // compensated sums allowed, bits 1.. = number of local entries in the item
+  // (the local / remote boundary of the item's range).
   const int32_t sdeg = o.short_degree > 0 ? o.short_degree : 32;
   out->items_short.clear();
   out->items_long.clear();
+  out->items_short_r.clear();
+  out->items_long_r.clear();
   out->empty_rows.clear();
   out->hub_rows.clear();
   out->hub_chunk_ptr.assign(1, 0);
   int32_t nchunks = 0;
+  auto add = [&](std::vector<WorkItem> &all, std::vector<WorkItem> &rem, int32_t outr, int64_t b,
+                 int64_t e, int64_t mid, int32_t kflag) {
+    const int64_t nloc = std::max<int64_t>(0, std::min(e, mid) - b);
+    const WorkItem w{outr, (int32_t)b, (int32_t)e, (int32_t)((nloc << 1) | kflag)};
+    all.push_back(w);
+    if (mid < e) rem.push_back(w);
+  };
   for (int64_t i = 0; i < M; ++i) {
-    const int64_t b = cnt[i], e = cnt[i + 1], d = e - b;
+    const int64_t b = cnt[i], e = cnt[i + 1], d = e - b, mid = b + lcnt[i];
     if (d == 0) {
       out->empty_rows.push_back((int32_t)i);
       continue;
     }
     const int32_t kflag = d > kdeg ? 1 : 0;
     if (d <= sdeg) {
-      out->items_short.push_back({(int32_t)i, (int32_t)b, (int32_t)e, kflag});
+      add(out->items_short, out->items_short_r, (int32_t)i, b, e, mid, kflag);
     } else if (d <= hub) {
-      out->items_long.push_back({(int32_t)i, (int32_t)b, (int32_t)e, kflag});
+      add(out->items_long, out->items_long_r, (int32_t)i, b, e, mid, kflag);
     } else {
       for (int64_t s = b; s < e; s += chunk) {
-        out->items_long.push_back({-(nchunks + 1), (int32_t)s,
-                                   (int32_t)std::min<int64_t>(e, s + chunk), kflag});
+        add(out->items_long, out->items_long_r, -(nchunks + 1), s, std::min<int64_t>(e, s + chunk), mid,
+            kflag);
         ++nchunks;
       }
       out->hub_rows.push_back((int32_t)i);
@@ -504,6 +520,8 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
   };
   split(out->items_short, &out->xcd_short);
   split(out->items_long, &out->xcd_long);
+  split(out->items_short_r, &out->xcd_short_r);
+  split(out->items_long_r, &out->xcd_long_r);
   return COLA_OK;
 }
 

@@ -94,12 +94,19 @@ struct Vec<COLA_FP8_E4M3, 16> {
   using T = uint8_t;
   using Raw = uint4;
   static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const uint4 *>(p); }
+  // gfx950's v_cvt_pk_f32_fp8 decodes OCP E4M3 (checked against the format
+  // definition for all 256 codes), two bytes per instruction.
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[16]) {
     const uint32_t w[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int b = 0; b < 4; ++b) x[4 * i + b] = fp8e4m3_to_f32((w[i] >> (8 * b)) & 0xffu);
+    for (int i = 0; i < 4; ++i) {
+      const auto lo = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[i], false);
+      const auto hi = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[i], true);
+      x[4 * i] = lo[0];
+      x[4 * i + 1] = lo[1];
+      x[4 * i + 2] = hi[0];
+      x[4 * i + 3] = hi[1];
+    }
   }
 };
 template <>
@@ -130,6 +137,8 @@ struct RowArgs {
   int kahan;
   int rpw;        // teams (items in flight) per wave, power of two <= 64/G
   int xcd_remap;  // 1: block b walks XCD (b % 8)'s item range
+  int phase;      // 0: all entries; 1: local entries only; 2: remote entries
+                  // only, added to what phase 1 wrote
 };
 
 template <int DT, int EPL, int G, int NV, int U, bool KAHAN>
@@ -170,7 +179,9 @@ __global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
     const bool has = it < ie;
     int4 item = make_int4(0, 0, 0, 0);
     if (has) item = a.items[it];
-    const int b = item.y, e = item.z;
+    int b = item.y, e = item.z;
+    if (a.phase == 1) e = b + (item.w >> 1);
+    else if (a.phase == 2) b += item.w >> 1;
     // Compensated sums only in the KAHAN instantiation and on flagged items.
     const bool comp = KAHAN && (item.w & 1);
     float acc[NV][EPL], err[NV][EPL];
@@ -256,10 +267,10 @@ __global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
       bool add_old;
       if (item.x >= 0) {
         dst = a.C + (int64_t)item.x * a.ldC;
-        add_old = a.accumulate != 0;
+        add_old = a.accumulate != 0 || a.phase == 2;
       } else {
         dst = a.partials + (int64_t)(-item.x - 1) * a.pstride;
-        add_old = false;
+        add_old = a.phase == 2;
       }
 #pragma unroll
       for (int k = 0; k < NV; ++k) {

@@ -383,13 +383,14 @@ struct PlanBin {
 struct cola_plan_s {
   int32_t num_rows = 0, nodesPerPE = 0, U_r = 0, U_l = 0, max_dim = 0;
   int64_t nnz = 0, num_empty = 0, num_hubs = 0, num_chunks = 0;
-  PlanBin shortb, longb;
+  PlanBin shortb, longb;      // every item
+  PlanBin shortb_r, longb_r;  // items with remote entries (second phase)
   int32_t *cols = nullptr, *empty_rows = nullptr, *hub_rows = nullptr, *hub_ptr = nullptr,
           *remote_cols = nullptr;
   float *vals = nullptr, *partials = nullptr;
   int64_t pstride = 0;
   int64_t device_bytes = 0;
-  cola_plan_tuning tune{0, 0, 1, 8, 1};
+  cola_plan_tuning tune{0, 0, 1, 8, 1, 0};
 };
 
 static void free_zip(cola_zip_s *z) {
@@ -405,7 +406,7 @@ static void free_zip(cola_zip_s *z) {
 
 static void free_plan(cola_plan_s *p) {
   if (!p) return;
-  for (PlanBin *b : {&p->shortb, &p->longb}) {
+  for (PlanBin *b : {&p->shortb, &p->longb, &p->shortb_r, &p->longb_r}) {
     (void)hipFree(b->items);
     (void)hipFree(b->xcd);
   }
@@ -504,34 +505,44 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
     if (!cola_dev::pick_shape(c.dtype, w, vec_ok, &sh))
       return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: dim " + std::to_string(c.dim) + " not supported");
     const int S = 64 / sh.g;
-    for (const PlanBin *bin : {&p->longb, &p->shortb}) {
-      if (bin->n == 0) continue;
-      cola_dev::RowArgs a;
-      a.items = bin->items;
-      a.xcd_begin = bin->xcd;
-      a.cols = p->cols;
-      a.vals = p->vals;
-      a.Bl = c.Bl ? (const char *)c.Bl + (size_t)c0 * eb : nullptr;
-      a.ldB = c.ldB;
-      a.sl = c.sl;
-      a.Bp = c.Bp ? (const char *)c.Bp + (size_t)c0 * eb : nullptr;
-      a.ldP = c.ldP;
-      a.sp = c.sp;
-      a.C = c.C + c0;
-      a.ldC = c.ldC;
-      a.partials = p->partials ? p->partials + c0 : nullptr;
-      a.pstride = p->pstride;
-      a.dim = w;
-      a.accumulate = accumulate;
-      a.kahan = kahan;
-      const int req = bin == &p->shortb ? (t.rpw_short > 0 ? t.rpw_short : S) : (t.rpw_long > 0 ? t.rpw_long : 1);
-      a.rpw = clamp_rpw(req, S);
-      a.xcd_remap = t.xcd_remap ? 1 : 0;
-      const int64_t waves = (bin->n + a.rpw - 1) / a.rpw;
-      int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bpc);
-      blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
-      if (!launch_rows(c.dtype, sh, a, (int)blocks, s))
-        return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: no kernel for this dim");
+    // phase 0: every entry at once; with split_remote, phase 1 (local
+    // entries over this partition's B slice) then phase 2 (remote entries
+    // over the packed buffer, added on top).
+    const bool split = t.split_remote && p->U_r > 0;
+    for (int phase : {split ? 1 : 0, split ? 2 : -1}) {
+      if (phase < 0) continue;
+      const PlanBin *bins[2] = {phase == 2 ? &p->longb_r : &p->longb, phase == 2 ? &p->shortb_r : &p->shortb};
+      for (int bi = 0; bi < 2; ++bi) {
+        const PlanBin *bin = bins[bi];
+        if (bin->n == 0) continue;
+        cola_dev::RowArgs a;
+        a.items = bin->items;
+        a.xcd_begin = bin->xcd;
+        a.cols = p->cols;
+        a.vals = p->vals;
+        a.Bl = c.Bl ? (const char *)c.Bl + (size_t)c0 * eb : nullptr;
+        a.ldB = c.ldB;
+        a.sl = c.sl;
+        a.Bp = c.Bp ? (const char *)c.Bp + (size_t)c0 * eb : nullptr;
+        a.ldP = c.ldP;
+        a.sp = c.sp;
+        a.C = c.C + c0;
+        a.ldC = c.ldC;
+        a.partials = p->partials ? p->partials + c0 : nullptr;
+        a.pstride = p->pstride;
+        a.dim = w;
+        a.accumulate = accumulate;
+        a.kahan = kahan;
+        const int req = bi == 1 ? (t.rpw_short > 0 ? t.rpw_short : S) : (t.rpw_long > 0 ? t.rpw_long : 1);
+        a.rpw = clamp_rpw(req, S);
+        a.xcd_remap = t.xcd_remap ? 1 : 0;
+        a.phase = phase;
+        const int64_t waves = (bin->n + a.rpw - 1) / a.rpw;
+        int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bpc);
+        blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
+        if (!launch_rows(c.dtype, sh, a, (int)blocks, s))
+          return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: no kernel for this dim");
+      }
     }
   }
   if (p->num_hubs > 0) {
@@ -751,10 +762,15 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
     return r;
   };
   std::vector<int4> is = to_int4(hp.items_short), il = to_int4(hp.items_long);
+  std::vector<int4> isr = to_int4(hp.items_short_r), ilr = to_int4(hp.items_long_r);
   p->shortb.n = (int64_t)is.size();
   p->longb.n = (int64_t)il.size();
+  p->shortb_r.n = (int64_t)isr.size();
+  p->longb_r.n = (int64_t)ilr.size();
   if ((st = upload(is, &p->shortb.items)) || (st = upload(hp.xcd_short, &p->shortb.xcd)) ||
       (st = upload(il, &p->longb.items)) || (st = upload(hp.xcd_long, &p->longb.xcd)) ||
+      (st = upload(isr, &p->shortb_r.items)) || (st = upload(hp.xcd_short_r, &p->shortb_r.xcd)) ||
+      (st = upload(ilr, &p->longb_r.items)) || (st = upload(hp.xcd_long_r, &p->longb_r.xcd)) ||
       (st = upload(hp.cols, &p->cols)) || (st = upload(hp.vals, &p->vals)) ||
       (st = upload(hp.empty_rows, &p->empty_rows)) || (st = upload(hp.hub_rows, &p->hub_rows)) ||
       (st = upload(hp.hub_chunk_ptr, &p->hub_ptr)) || (st = upload(hp.remote_cols, &p->remote_cols))) {
@@ -769,7 +785,7 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
       return st;
     }
   }
-  p->device_bytes = (int64_t)((is.size() + il.size()) * sizeof(int4) + 18 * 4 + hp.cols.size() * 4 +
+  p->device_bytes = (int64_t)((is.size() + il.size() + isr.size() + ilr.size()) * sizeof(int4) + 36 * 4 + hp.cols.size() * 4 +
                               hp.vals.size() * 4 + hp.empty_rows.size() * 4 + hp.hub_rows.size() * 4 +
                               hp.hub_chunk_ptr.size() * 4 + hp.remote_cols.size() * 4 +
                               (size_t)p->num_chunks * p->pstride * 4);
@@ -795,7 +811,8 @@ cola_status cola_plan_get_info(cola_plan p, cola_plan_info *info) {
 cola_status cola_plan_set_tuning(cola_plan p, const cola_plan_tuning *t) {
   if (!p || !t) return fail(COLA_ERR_INVALID, "plan_set_tuning: NULL argument");
   if (t->col_tile < 0 || t->rpw_short < 0 || t->rpw_short > 64 || t->rpw_long < 0 || t->rpw_long > 64 ||
-      t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || (t->xcd_remap != 0 && t->xcd_remap != 1))
+      t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || (t->xcd_remap != 0 && t->xcd_remap != 1) ||
+      (t->split_remote != 0 && t->split_remote != 1))
     return fail(COLA_ERR_INVALID, "plan_set_tuning: parameter out of range");
   p->tune = *t;
   return COLA_OK;
@@ -892,12 +909,17 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
       t.xcd_remap = x;
       cands.push_back(t);
     }
+    for (int x : {0, 1}) {
+      cola_plan_tuning t = best;
+      t.split_remote = x;
+      cands.push_back(t);
+    }
     for (const auto &t : cands) {
       float ms = 0.f;
       if ((st = time_of(t, &ms))) break;
       if (verbose)
-        fprintf(stderr, "[autotune] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d: %.4f ms\n",
-                t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.xcd_remap, ms);
+        fprintf(stderr, "[autotune] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d split=%d: %.4f ms\n",
+                t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.xcd_remap, t.split_remote, ms);
       if (ms < bms * 0.99f) {
         bms = ms;
         best = t;

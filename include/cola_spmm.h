@@ -261,6 +261,9 @@ typedef struct {
   int32_t blocks_per_cu; /* grid size in 256-thread blocks per CU (0 = 8)   */
   int32_t xcd_remap;     /* 1: each XCD walks a contiguous row range
                             (default); 0: plain block order                 */
+  int32_t split_remote;  /* 1: local entries first (B_p only), then remote
+                            entries (packed only) added on top; 0: both in
+                            one pass (default)                              */
 } cola_plan_tuning;
 cola_status cola_plan_set_tuning(cola_plan plan, const cola_plan_tuning *t);
 cola_status cola_plan_get_tuning(cola_plan plan, cola_plan_tuning *t);

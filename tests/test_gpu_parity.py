@@ -361,10 +361,11 @@ def test_full_size_partition_checksum():
     assert rel_fro(Ch[sample], refs) <= FP32_TOL
 
 
-TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1),
-           dict(col_tile=32, rpw_short=1, rpw_long=2, blocks_per_cu=2, xcd_remap=0),
-           dict(col_tile=16, rpw_short=2, rpw_long=4, blocks_per_cu=16, xcd_remap=1),
-           dict(col_tile=48, rpw_short=64, rpw_long=64, blocks_per_cu=1, xcd_remap=0)]
+TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1, split_remote=0),
+           dict(col_tile=32, rpw_short=1, rpw_long=2, blocks_per_cu=2, xcd_remap=0, split_remote=1),
+           dict(col_tile=16, rpw_short=2, rpw_long=4, blocks_per_cu=16, xcd_remap=1, split_remote=0),
+           dict(col_tile=48, rpw_short=64, rpw_long=64, blocks_per_cu=1, xcd_remap=0, split_remote=1),
+           dict(col_tile=0, rpw_short=4, rpw_long=1, blocks_per_cu=4, xcd_remap=1, split_remote=1)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
@@ -387,8 +388,14 @@ def test_tunings_give_the_same_result(c1, tuning, dtype):
         plan.spmm(C, Bq[:c1.P], packed, dim=dim, dtype=dtype, flags=cola.OVERWRITE,
                   scales_local=None if sc is None else sc[:c1.P], packed_scales=psc)
         outs.append(C.cpu().numpy())
+    # accumulate mode on top of a non-zero C
+    C = torch.full((c1.P, dim), 3.0, dtype=torch.float32, device=B.device)
+    plan.spmm(C, Bq[:c1.P], packed, dim=dim, dtype=dtype, flags=cola.ACCUMULATE,
+              scales_local=None if sc is None else sc[:c1.P], packed_scales=psc)
+    acc = C.cpu().numpy()
     torch.cuda.synchronize()
     assert rel_fro(outs[1], outs[0].astype(np.float64)) <= 1e-6
+    assert rel_fro(acc - 3.0, outs[0].astype(np.float64)) <= 2e-6
     if dtype == cola.F32:
         assert rel_fro(outs[1][:c1.M], c1.ref) <= FP32_TOL
 
