@@ -190,35 +190,52 @@ __global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
 #pragma unroll
       for (int q = 0; q < EPL; ++q) acc[k][q] = err[k][q] = 0.f;
 
+    // Entries are taken in chunks of tsize: one coalesced load of (column,
+    // value) per team lane, the next chunk prefetched while this one is used.
+    int mc = 0, mc_next = 0;
+    float mv = 0.f, mv_next = 0.f;
+    if (b + tl < e) {
+      mc = a.cols[b + tl];
+      mv = a.vals[b + tl];
+    }
     for (int base = b; base < e; base += tsize) {
       const int n = min(tsize, e - base);
-      int mc = 0;
-      float mv = 0.f;
-      if (tl < n) {
-        mc = a.cols[base + tl];
-        mv = a.vals[base + tl];
+      const int nb = base + tsize;
+      if (nb + tl < e) {
+        mc_next = a.cols[nb + tl];
+        mv_next = a.vals[nb + tl];
       }
       for (int j = 0; j < n; j += gpt * U) {
+        // Broadcast the batch's (column, value) pairs first, then issue every
+        // load.  Slots past the end re-read the last entry's row (a valid,
+        // just-fetched address) with weight 0, so no load is predicated.
+        int cu[U];
+        float vu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int ei = j + u * gpt + tsub;
+          const int src = (tbase + min(ei, n - 1)) << 2;
+          cu[u] = __builtin_amdgcn_ds_bpermute(src, mc);
+          vu[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv)));
+        }
         Raw raw[U][NV];
         float av[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int ei = j + u * gpt + tsub;
-          const bool ok = ei < n;
-          const int src = (tbase + (ok ? ei : 0)) << 2;
-          const int c = __builtin_amdgcn_ds_bpermute(src, mc);
-          float v = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv)));
-          const T *rp = c >= 0 ? Bl + (int64_t)c * a.ldB : Bp + (int64_t)(~c) * a.ldP;
-          if constexpr (DT == COLA_FP8_E4M3) {
-            if (ok) v *= (c >= 0 ? a.sl[c] : a.sp[~c]);
-          }
-          av[u] = ok ? v : 0.f;
+          const int c = cu[u];
+          const bool loc = c >= 0;
+          const T *rp = (loc ? Bl : Bp) + (int64_t)(loc ? c : ~c) * (loc ? a.ldB : a.ldP);
 #pragma unroll
           for (int k = 0; k < NV; ++k) {
             const int col = (k * G + gl) * EPL;
-            if (ok && col < a.dim) raw[u][k] = V::load(rp + col);
-            else raw[u][k] = Raw{};
+            raw[u][k] = V::load(rp + (col < a.dim ? col : 0));
           }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          float v = vu[u];
+          if constexpr (DT == COLA_FP8_E4M3) v *= (cu[u] >= 0 ? a.sl[cu[u]] : a.sp[~cu[u]]);
+          av[u] = (j + u * gpt + tsub < n) ? v : 0.f;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -241,6 +258,8 @@ __global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
             }
           }
       }
+      mc = mc_next;
+      mv = mv_next;
     }
     // Sum the groups of each team (xor offsets below tsize stay in the team).
     for (int off = G; off < tsize; off <<= 1) {
