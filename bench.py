@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--tuning", default="auto",
+                    help='"auto" (autotune before timing) or a JSON object of plan tuning fields')
+    ap.add_argument("--save-tuning", default=None, help="write the FP32 tuning used to this JSON file")
     ap.add_argument("--copy-calibrate-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -89,8 +92,16 @@ def main():
         del src, dst
 
     plan.pack(B, packed)
-    tuning, tune_ms = plan.autotune(C, B_local, packed, iters=3)
-    log(f"autotuned f32: {tuning} ({tune_ms:.3f} ms)")
+    if args.tuning == "auto":
+        tuning, tune_ms = plan.autotune(C, B_local, packed, iters=3)
+        log(f"autotuned f32: {tuning} ({tune_ms:.3f} ms)")
+    else:
+        plan.tuning = json.loads(args.tuning)
+        tuning = plan.tuning
+        log(f"fixed f32 tuning: {tuning}")
+    if args.save_tuning:
+        with open(args.save_tuning, "w") as f:
+            json.dump(tuning, f)
 
     def step():
         plan.pack(B, packed)

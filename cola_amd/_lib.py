@@ -57,7 +57,8 @@ class PlanOpts(ctypes.Structure):
 
 class PlanTuning(ctypes.Structure):
     _fields_ = [("col_tile", c_i32), ("rpw_short", c_i32), ("rpw_long", c_i32), ("blocks_per_cu", c_i32),
-                ("xcd_remap", c_i32), ("split_remote", c_i32)]
+                ("xcd_remap", c_i32), ("split_remote", c_i32),
+                ("unroll", c_i32)]
 
 
 class PlanInfo(ctypes.Structure):

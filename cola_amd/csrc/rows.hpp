@@ -353,29 +353,39 @@ inline bool pick_shape(int dt, int dim, bool vec_ok, Shape *s) {
   return true;
 }
 
-// Loads in flight per group: 8, fewer when a lane already holds several
-// vectors per entry.
-template <int NV>
+// Loads in flight per group: U8 (8 or 16), fewer when a lane already holds
+// several vectors per entry.
+template <int NV, int U8>
 struct Unroll {
-  static constexpr int value = NV >= 8 ? 1 : 8 / NV;
+  static constexpr int value = NV >= U8 ? 1 : U8 / NV;
 };
 
 // Launches one row kernel; returns false if no instantiation matches.
-bool launch_rows_f32(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s);
-bool launch_rows_bf16(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s);
-bool launch_rows_f16(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s);
-bool launch_rows_fp8(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s);
+bool launch_rows_f32(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s,
+                      bool unroll16);
+bool launch_rows_bf16(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s,
+                      bool unroll16);
+bool launch_rows_f16(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s,
+                      bool unroll16);
+bool launch_rows_fp8(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s,
+                      bool unroll16);
 
 template <int DT, int EV>
-bool launch_rows_impl(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s) {
+bool launch_rows_impl(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s,
+                      bool unroll16) {
+#define COLA_ROWS_LAUNCH(E, G_, NV_, U8)                                                          \
+  do {                                                                                            \
+    if (a.kahan)                                                                                  \
+      hipLaunchKernelGGL((plan_rows_kernel<DT, E, G_, NV_, Unroll<NV_, U8>::value, true>),         \
+                         dim3(blocks), dim3(threads), 0, s, a);                                   \
+    else                                                                                          \
+      hipLaunchKernelGGL((plan_rows_kernel<DT, E, G_, NV_, Unroll<NV_, U8>::value, false>),        \
+                         dim3(blocks), dim3(threads), 0, s, a);                                   \
+  } while (0)
 #define COLA_ROWS_CASE(E, G_, NV_)                                                                \
   if (sh.epl == E && sh.g == G_ && sh.nv == NV_) {                                                \
-    if (a.kahan)                                                                                  \
-      hipLaunchKernelGGL((plan_rows_kernel<DT, E, G_, NV_, Unroll<NV_>::value, true>), dim3(blocks), \
-                         dim3(threads), 0, s, a);                                                 \
-    else                                                                                          \
-      hipLaunchKernelGGL((plan_rows_kernel<DT, E, G_, NV_, Unroll<NV_>::value, false>),           \
-                         dim3(blocks), dim3(threads), 0, s, a);                                   \
+    if (unroll16 && NV_ == 1 && E > 1) COLA_ROWS_LAUNCH(E, G_, NV_, 16);                          \
+    else COLA_ROWS_LAUNCH(E, G_, NV_, 8);                                                         \
     return true;                                                                                  \
   }
   COLA_ROWS_CASE(EV, 2, 1)
@@ -391,6 +401,7 @@ bool launch_rows_impl(const Shape &sh, const RowArgs &a, int blocks, int threads
   COLA_ROWS_CASE(1, 64, 4)
   COLA_ROWS_CASE(1, 64, 8)
 #undef COLA_ROWS_CASE
+#undef COLA_ROWS_LAUNCH
   return false;
 }
 

@@ -4,7 +4,8 @@
 #include "rows.hpp"
 
 namespace cola_dev {
-bool launch_rows_f16(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s) {
-  return launch_rows_impl<COLA_F16, 8>(sh, a, blocks, threads, s);
+bool launch_rows_f16(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s,
+                      bool unroll16) {
+  return launch_rows_impl<COLA_F16, 8>(sh, a, blocks, threads, s, unroll16);
 }
 }  // namespace cola_dev

@@ -4,7 +4,8 @@
 #include "rows.hpp"
 
 namespace cola_dev {
-bool launch_rows_fp8(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s) {
-  return launch_rows_impl<COLA_FP8_E4M3, 16>(sh, a, blocks, threads, s);
+bool launch_rows_fp8(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s,
+                      bool unroll16) {
+  return launch_rows_impl<COLA_FP8_E4M3, 16>(sh, a, blocks, threads, s, unroll16);
 }
 }  // namespace cola_dev

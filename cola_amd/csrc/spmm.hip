@@ -390,7 +390,7 @@ struct cola_plan_s {
   float *vals = nullptr, *partials = nullptr;
   int64_t pstride = 0;
   int64_t device_bytes = 0;
-  cola_plan_tuning tune{0, 0, 1, 8, 1, 0};
+  cola_plan_tuning tune{0, 0, 1, 8, 1, 0, 8};
 };
 
 static void free_zip(cola_zip_s *z) {
@@ -438,12 +438,12 @@ void launch_ppl0(float *out, const float *in, const float *packed, const int32_t
 
 int vec_elems(int dt) { return dt == COLA_F32 ? 4 : (dt == COLA_FP8_E4M3 ? 16 : 8); }
 
-bool launch_rows(int dt, const Shape &sh, const cola_dev::RowArgs &a, int blocks, hipStream_t s) {
+bool launch_rows(int dt, const Shape &sh, const cola_dev::RowArgs &a, int blocks, hipStream_t s, bool u16) {
   switch (dt) {
-    case COLA_F32: return cola_dev::launch_rows_f32(sh, a, blocks, 256, s);
-    case COLA_BF16: return cola_dev::launch_rows_bf16(sh, a, blocks, 256, s);
-    case COLA_F16: return cola_dev::launch_rows_f16(sh, a, blocks, 256, s);
-    case COLA_FP8_E4M3: return cola_dev::launch_rows_fp8(sh, a, blocks, 256, s);
+    case COLA_F32: return cola_dev::launch_rows_f32(sh, a, blocks, 256, s, u16);
+    case COLA_BF16: return cola_dev::launch_rows_bf16(sh, a, blocks, 256, s, u16);
+    case COLA_F16: return cola_dev::launch_rows_f16(sh, a, blocks, 256, s, u16);
+    case COLA_FP8_E4M3: return cola_dev::launch_rows_fp8(sh, a, blocks, 256, s, u16);
   }
   return false;
 }
@@ -540,7 +540,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
         const int64_t waves = (bin->n + a.rpw - 1) / a.rpw;
         int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bpc);
         blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
-        if (!launch_rows(c.dtype, sh, a, (int)blocks, s))
+        if (!launch_rows(c.dtype, sh, a, (int)blocks, s, t.unroll == 16))
           return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: no kernel for this dim");
       }
     }
@@ -812,7 +812,7 @@ cola_status cola_plan_set_tuning(cola_plan p, const cola_plan_tuning *t) {
   if (!p || !t) return fail(COLA_ERR_INVALID, "plan_set_tuning: NULL argument");
   if (t->col_tile < 0 || t->rpw_short < 0 || t->rpw_short > 64 || t->rpw_long < 0 || t->rpw_long > 64 ||
       t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || (t->xcd_remap != 0 && t->xcd_remap != 1) ||
-      (t->split_remote != 0 && t->split_remote != 1))
+      (t->split_remote != 0 && t->split_remote != 1) || (t->unroll != 0 && t->unroll != 8 && t->unroll != 16))
     return fail(COLA_ERR_INVALID, "plan_set_tuning: parameter out of range");
   p->tune = *t;
   return COLA_OK;
@@ -914,12 +914,18 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
       t.split_remote = x;
       cands.push_back(t);
     }
+    for (int u : {8, 16}) {
+      cola_plan_tuning t = best;
+      t.unroll = u;
+      cands.push_back(t);
+    }
     for (const auto &t : cands) {
       float ms = 0.f;
       if ((st = time_of(t, &ms))) break;
       if (verbose)
-        fprintf(stderr, "[autotune] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d split=%d: %.4f ms\n",
-                t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.xcd_remap, t.split_remote, ms);
+        fprintf(stderr,
+                "[autotune] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d split=%d unroll=%d: %.4f ms\n",
+                t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.xcd_remap, t.split_remote, t.unroll, ms);
       if (ms < bms * 0.99f) {
         bms = ms;
         best = t;

@@ -264,6 +264,8 @@ typedef struct {
   int32_t split_remote;  /* 1: local entries first (B_p only), then remote
                             entries (packed only) added on top; 0: both in
                             one pass (default)                              */
+  int32_t unroll;        /* row-gather loads in flight per lane group: 8
+                            (default, also 0) or 16                         */
 } cola_plan_tuning;
 cola_status cola_plan_set_tuning(cola_plan plan, const cola_plan_tuning *t);
 cola_status cola_plan_get_tuning(cola_plan plan, cola_plan_tuning *t);

@@ -361,11 +361,12 @@ def test_full_size_partition_checksum():
     assert rel_fro(Ch[sample], refs) <= FP32_TOL
 
 
-TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1, split_remote=0),
+TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=8),
            dict(col_tile=32, rpw_short=1, rpw_long=2, blocks_per_cu=2, xcd_remap=0, split_remote=1),
            dict(col_tile=16, rpw_short=2, rpw_long=4, blocks_per_cu=16, xcd_remap=1, split_remote=0),
            dict(col_tile=48, rpw_short=64, rpw_long=64, blocks_per_cu=1, xcd_remap=0, split_remote=1),
-           dict(col_tile=0, rpw_short=4, rpw_long=1, blocks_per_cu=4, xcd_remap=1, split_remote=1)]
+           dict(col_tile=0, rpw_short=4, rpw_long=1, blocks_per_cu=4, xcd_remap=1, split_remote=1, unroll=16),
+           dict(col_tile=32, rpw_short=8, rpw_long=2, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=16)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
