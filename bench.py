@@ -140,7 +140,10 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "plan_rows_f32_rmat22_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        # Counter-measured traffic holds only for the schedule it was measured with.
+        if tj.get("tuning") == tuning and N == 128 and args.log_n == 22 and args.rank == 0:
+            traffic = tj.get("hbm_bytes_per_launch")
 
     extras = {}
     if not args.no_extras:
