@@ -61,6 +61,15 @@ class PlanTuning(ctypes.Structure):
                 ("unroll", c_i32)]
 
 
+class BatchProblem(ctypes.Structure):
+    _fields_ = [("plan", c_vp), ("dim", c_i32), ("B", c_vp), ("ld_B", c_i64), ("scales", c_vp), ("packed", c_vp),
+                ("ld_packed", c_i64), ("packed_scales", c_vp), ("C", c_vp), ("ld_C", c_i64)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("col_tile", c_i32), ("rpw_short", c_i32), ("rpw_long", c_i32)]
+
+
 class PlanInfo(ctypes.Structure):
     _fields_ = [("num_rows", c_i64), ("num_items", c_i64), ("num_hub_rows", c_i64),
                 ("num_hub_chunks", c_i64), ("nnz", c_i64), ("U_r", c_i64), ("U_l", c_i64),
@@ -103,6 +112,14 @@ SIGNATURES = {
     "cola_plan_destroy": [c_vp],
     "cola_spmm_partition": [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_int, c_int,
                             c_int, c_vp],
+    "cola_batch_create": [ctypes.POINTER(BatchProblem), c_int, c_int, c_int, ctypes.POINTER(c_vp)],
+    "cola_batch_set_tuning": [c_vp, c_int, ctypes.POINTER(BatchTuning), c_int],
+    "cola_batch_get_tuning": [c_vp, c_int, ctypes.POINTER(BatchTuning), ctypes.POINTER(c_int)],
+    "cola_batch_autotune": [c_vp, c_int, c_vp, ctypes.POINTER(ctypes.c_float)],
+    "cola_batch_run": [c_vp, c_vp],
+    "cola_batch_capture": [c_vp, c_vp],
+    "cola_batch_replay": [c_vp, c_vp],
+    "cola_batch_destroy": [c_vp],
     "cola_quantize_rows": [c_vp, c_i64, c_i64, c_int, c_int, c_vp, c_i64, c_vp, c_vp],
     "cola_error_norms": [c_vp, c_i64, c_vp, c_i64, c_i64, c_int, c_vp, c_vp],
 }

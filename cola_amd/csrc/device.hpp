@@ -1,0 +1,55 @@
+// Device-side handle structures and host helpers shared by spmm.hip (plan
+// path) and batch.hip (batched / served path).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "cola_internal.hpp"
+
+namespace cola_devhost {
+cola_status hip_check(hipError_t e, const char *what);
+int device_cu_count();
+size_t dtype_bytes(int dt);
+int vec_elems(int dt);  // elements per 16-byte lane vector
+bool aligned(const void *p, size_t bytes);
+
+template <typename T>
+cola_status upload(const std::vector<T> &v, T **d) {
+  *d = nullptr;
+  if (v.empty()) return COLA_OK;
+  cola_status st = hip_check(hipMalloc((void **)d, v.size() * sizeof(T)), "hipMalloc");
+  if (st) return st;
+  return hip_check(hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice),
+                   "hipMemcpy H2D");
+}
+}  // namespace cola_devhost
+
+struct cola_zip_s {
+  int32_t *ptr = nullptr, *idx = nullptr, *zip = nullptr, *pack_idx = nullptr,
+          *uniq = nullptr;
+  float *val = nullptr;
+  int64_t nptr = 0, nnz = 0, n_unique = 0;
+};
+
+struct PlanBin {
+  int4 *items = nullptr;
+  int32_t *xcd = nullptr;  // 9 entries: item range per XCD
+  int64_t n = 0;
+};
+
+struct cola_plan_s {
+  int32_t num_rows = 0, nodesPerPE = 0, U_r = 0, U_l = 0, max_dim = 0;
+  int64_t nnz = 0, num_empty = 0, num_hubs = 0, num_chunks = 0;
+  PlanBin shortb, longb;      // every item
+  PlanBin shortb_r, longb_r;  // items with remote entries (second phase)
+  PlanBin emptyb;             // rows without entries as empty items (batched overwrite)
+  int32_t *cols = nullptr, *empty_rows = nullptr, *hub_rows = nullptr, *hub_ptr = nullptr,
+          *remote_cols = nullptr;
+  float *vals = nullptr, *partials = nullptr;
+  int64_t pstride = 0;
+  int64_t device_bytes = 0;
+  cola_plan_tuning tune{0, 0, 1, 8, 1, 0, 8};
+};

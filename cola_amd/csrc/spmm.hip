@@ -34,6 +34,43 @@
 
 #include "cola_internal.hpp"
 #include "rows.hpp"
+#include "device.hpp"
+
+using namespace cola_devhost;
+
+namespace cola_devhost {
+using cola::fail;
+bool aligned(const void *p, size_t bytes) { return ((uintptr_t)p % bytes) == 0; }
+
+cola_status hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return COLA_OK;
+  return fail(COLA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int device_cu_count() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cached[dev] == 0) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      n = 256;
+    cached[dev] = n;
+  }
+  return cached[dev];
+}
+
+size_t dtype_bytes(int dt) {
+  switch (dt) {
+    case COLA_F32: return 4;
+    case COLA_BF16: case COLA_F16: return 2;
+    case COLA_FP8_E4M3: return 1;
+  }
+  return 0;
+}
+
+int vec_elems(int dt) { return dt == COLA_F32 ? 4 : (dt == COLA_FP8_E4M3 ? 16 : 8); }
+}  // namespace cola_devhost
 
 using cola::fail;
 
@@ -67,14 +104,6 @@ __device__ __forceinline__ uint32_t f32_to_fp8e4m3_rne(float f) {
   return sign | ((uint32_t)e << 3) | m;
 }
 
-size_t dtype_bytes(int dt) {
-  switch (dt) {
-    case COLA_F32: return 4;
-    case COLA_BF16: case COLA_F16: return 2;
-    case COLA_FP8_E4M3: return 1;
-  }
-  return 0;
-}
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -329,70 +358,11 @@ __global__ __launch_bounds__(256) void error_norms_kernel(const float *__restric
   }
 }
 
-// ---------------------------------------------------------------------------
-// Host helpers.
-// ---------------------------------------------------------------------------
-bool aligned(const void *p, size_t bytes) { return ((uintptr_t)p % bytes) == 0; }
-
-cola_status hip_check(hipError_t e, const char *what) {
-  if (e == hipSuccess) return COLA_OK;
-  return fail(COLA_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-int device_cu_count() {
-  static int cached[64] = {0};
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (cached[dev] == 0) {
-    int n = 0;
-    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-      n = 256;
-    cached[dev] = n;
-  }
-  return cached[dev];
-}
-
-template <typename T>
-cola_status upload(const std::vector<T> &v, T **d) {
-  *d = nullptr;
-  if (v.empty()) return COLA_OK;
-  cola_status st = hip_check(hipMalloc((void **)d, v.size() * sizeof(T)), "hipMalloc");
-  if (st) return st;
-  return hip_check(hipMemcpy(*d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice),
-                   "hipMemcpy H2D");
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // Handles.
 // ---------------------------------------------------------------------------
-struct cola_zip_s {
-  int32_t *ptr = nullptr, *idx = nullptr, *zip = nullptr, *pack_idx = nullptr,
-          *uniq = nullptr;
-  float *val = nullptr;
-  int64_t nptr = 0, nnz = 0, n_unique = 0;
-};
-
-struct PlanBin {
-  int4 *items = nullptr;
-  int32_t *xcd = nullptr;
-  int64_t n = 0;
-};
-
-struct cola_plan_s {
-  int32_t num_rows = 0, nodesPerPE = 0, U_r = 0, U_l = 0, max_dim = 0;
-  int64_t nnz = 0, num_empty = 0, num_hubs = 0, num_chunks = 0;
-  PlanBin shortb, longb;      // every item
-  PlanBin shortb_r, longb_r;  // items with remote entries (second phase)
-  int32_t *cols = nullptr, *empty_rows = nullptr, *hub_rows = nullptr, *hub_ptr = nullptr,
-          *remote_cols = nullptr;
-  float *vals = nullptr, *partials = nullptr;
-  int64_t pstride = 0;
-  int64_t device_bytes = 0;
-  cola_plan_tuning tune{0, 0, 1, 8, 1, 0, 8};
-};
-
 static void free_zip(cola_zip_s *z) {
   if (!z) return;
   (void)hipFree(z->ptr);
@@ -406,7 +376,7 @@ static void free_zip(cola_zip_s *z) {
 
 static void free_plan(cola_plan_s *p) {
   if (!p) return;
-  for (PlanBin *b : {&p->shortb, &p->longb, &p->shortb_r, &p->longb_r}) {
+  for (PlanBin *b : {&p->shortb, &p->longb, &p->shortb_r, &p->longb_r, &p->emptyb}) {
     (void)hipFree(b->items);
     (void)hipFree(b->xcd);
   }
@@ -435,8 +405,6 @@ void launch_ppl0(float *out, const float *in, const float *packed, const int32_t
   hipLaunchKernelGGL((ppl0_kernel<EPL, G, NV>), dim3((unsigned)blocks), dim3(256), 0, s, out,
                      in, packed, rp, ci, ri, vl, cp, rr, pi, vr, llb, rlb, L, R, dim, peid);
 }
-
-int vec_elems(int dt) { return dt == COLA_F32 ? 4 : (dt == COLA_FP8_E4M3 ? 16 : 8); }
 
 bool launch_rows(int dt, const Shape &sh, const cola_dev::RowArgs &a, int blocks, hipStream_t s, bool u16) {
   switch (dt) {
@@ -776,6 +744,15 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
       (st = upload(hp.hub_chunk_ptr, &p->hub_ptr)) || (st = upload(hp.remote_cols, &p->remote_cols))) {
     free_plan(p);
     return st;
+  }
+  {
+    std::vector<int4> ie(hp.empty_rows.size());
+    for (size_t i = 0; i < ie.size(); ++i) ie[i] = make_int4(hp.empty_rows[i], 0, 0, 0);
+    p->emptyb.n = (int64_t)ie.size();
+    if ((st = upload(ie, &p->emptyb.items))) {
+      free_plan(p);
+      return st;
+    }
   }
   if (p->num_chunks > 0) {
     st = hip_check(hipMalloc((void **)&p->partials, (size_t)p->num_chunks * p->pstride * sizeof(float)),

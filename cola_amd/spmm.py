@@ -209,6 +209,64 @@ class Plan:
         return {f: getattr(best, f) for f in self.TUNING_FIELDS}, ms.value
 
 
+class Batch:
+    """Many independent SpMMs in one persistent launch.
+
+    problems: list of dicts with keys plan, B, C and optionally dim, scales,
+    packed, packed_scales (device tensors; kept alive by the batch)."""
+
+    def __init__(self, problems, dtype=F32, flags=OVERWRITE):
+        self._keep = []
+        arr = (_lib.BatchProblem * max(1, len(problems)))()
+        for k, pr in enumerate(problems):
+            B, C = pr["B"], pr["C"]
+            packed = pr.get("packed")
+            arr[k].plan = pr["plan"].handle.value
+            arr[k].dim = int(pr.get("dim", C.shape[1]))
+            arr[k].B = B.data_ptr() if B is not None else None
+            arr[k].ld_B = B.stride(0) if B is not None else arr[k].dim
+            arr[k].scales = pr["scales"].data_ptr() if pr.get("scales") is not None else None
+            arr[k].packed = packed.data_ptr() if packed is not None else None
+            arr[k].ld_packed = packed.stride(0) if packed is not None else arr[k].dim
+            arr[k].packed_scales = pr["packed_scales"].data_ptr() if pr.get("packed_scales") is not None else None
+            arr[k].C = C.data_ptr()
+            arr[k].ld_C = C.stride(0)
+            self._keep.append(pr)
+        h = ctypes.c_void_p()
+        check(lib().cola_batch_create(arr, len(problems), int(dtype), int(flags), ctypes.byref(h)))
+        self._h = h
+        self.n = len(problems)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.cola_batch_destroy(self._h)
+            self._h = None
+
+    def run(self, stream=None):
+        check(lib().cola_batch_run(self._h, _stream(stream)))
+
+    def capture(self, stream):
+        check(lib().cola_batch_capture(self._h, _stream(stream)))
+
+    def replay(self, stream=None):
+        check(lib().cola_batch_replay(self._h, _stream(stream)))
+
+    def autotune(self, iters=3, stream=None):
+        ms = ctypes.c_float()
+        check(lib().cola_batch_autotune(self._h, int(iters), _stream(stream), ctypes.byref(ms)))
+        return ms.value
+
+    def set_tuning(self, problem=-1, col_tile=0, rpw_short=0, rpw_long=0, blocks_per_cu=0):
+        t = _lib.BatchTuning(int(col_tile), int(rpw_short), int(rpw_long))
+        check(lib().cola_batch_set_tuning(self._h, int(problem), ctypes.byref(t), int(blocks_per_cu)))
+
+    def tuning(self, problem):
+        t = _lib.BatchTuning()
+        bpc = ctypes.c_int()
+        check(lib().cola_batch_get_tuning(self._h, int(problem), ctypes.byref(t), ctypes.byref(bpc)))
+        return dict(col_tile=t.col_tile, rpw_short=t.rpw_short, rpw_long=t.rpw_long, blocks_per_cu=bpc.value)
+
+
 def quantize_rows(src, dtype, dst=None, scales=None, stream=None):
     """FP32 rows -> storage dtype (RNE; FP8 with per-row scale max|x|/448)."""
     torch = _torch()

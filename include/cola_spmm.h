@@ -297,6 +297,59 @@ cola_status cola_spmm_partition(cola_plan plan, float *C, int64_t ld_C,
                                 int dim, cola_dtype dtype, int flags,
                                 void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Batched form: many independent SpMMs in one persistent launch.            */
+/* ------------------------------------------------------------------------ */
+
+/* One problem of a batch: C (+)= A·B for the plan's partition (a whole graph
+ * is the partition of nGPU = 1), with its own dim, B and C.  The batch keeps
+ * the pointers; the caller keeps the buffers alive and unchanged in size. */
+typedef struct {
+  cola_plan plan;
+  int32_t dim;                /* a multiple of the 16-byte vector width
+                                 (4 FP32, 8 BF16/FP16, 16 FP8), <= 64 vectors */
+  const void *B;              /* local rows, ld_B elements per row           */
+  int64_t ld_B;
+  const float *scales;        /* FP8 row scales of B, else NULL              */
+  const void *packed;         /* packed remote rows (partitions), or NULL    */
+  int64_t ld_packed;
+  const float *packed_scales;
+  float *C;                   /* FP32 output, ld_C per row                   */
+  int64_t ld_C;
+} cola_batch_problem;
+typedef struct cola_batch_s *cola_batch;
+typedef struct {
+  int32_t col_tile;  /* dense columns per pass (0 = all)                     */
+  int32_t rpw_short; /* short-row bin: rows per wave (0 = one per lane group) */
+  int32_t rpw_long;  /* long-row bin: rows per wave (0 = 1)                  */
+} cola_batch_tuning;
+
+/* Builds the batch's device work list (segments of every problem's degree
+ * bins cut into tasks of rows-per-wave items x column tiles) for one storage
+ * dtype; flags: COLA_OVERWRITE and/or COLA_KAHAN (else accumulate).  Hub-row
+ * workspace is allocated per problem, so problems sharing a plan do not race. */
+cola_status cola_batch_create(const cola_batch_problem *probs, int n,
+                              cola_dtype dtype, int flags, cola_batch *out);
+/* Sets one problem's schedule (problem = -1: all) and/or the grid size
+ * (blocks_per_cu > 0); rebuilds the work list and drops a captured graph. */
+cola_status cola_batch_set_tuning(cola_batch b, int problem,
+                                  const cola_batch_tuning *t, int blocks_per_cu);
+cola_status cola_batch_get_tuning(cola_batch b, int problem,
+                                  cola_batch_tuning *t, int *blocks_per_cu);
+/* Autotuner: coordinate descent over each dim class's column tile and
+ * rows-per-wave of both bins, then the grid size, timed with HIP events.
+ * The outputs hold the batch's result on return. */
+cola_status cola_batch_autotune(cola_batch b, int iters, void *stream,
+                                float *best_ms);
+/* One persistent launch over every task (+ one launch that reduces the
+ * hub rows of every problem). */
+cola_status cola_batch_run(cola_batch b, void *stream);
+/* Captures cola_batch_run into a hipGraph held by the batch (stream must not
+ * be the default stream); cola_batch_replay launches it. */
+cola_status cola_batch_capture(cola_batch b, void *stream);
+cola_status cola_batch_replay(cola_batch b, void *stream);
+cola_status cola_batch_destroy(cola_batch b);
+
 /* Row-wise conversion of FP32 rows to a storage dtype (round to nearest
  * even; FP8 rows get scale = max|row|/448 and are stored as x/scale).
  * dst has ld_dst elements per row; scales (FP8 only) has rows entries. */

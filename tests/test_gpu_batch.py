@@ -1,0 +1,166 @@
+"""Batched form: many SpMMs in one persistent launch, hipGraph replay,
+autotuning — checked against per-problem launches and the FP64 oracle."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import cola_amd as cola  # noqa: E402
+from oracle import oracle as orc  # noqa: E402
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a GPU")]
+
+TDT = {cola.F32: torch.float32, cola.BF16: torch.uint16, cola.F16: torch.uint16, cola.FP8_E4M3: torch.uint8}
+
+
+def rel_fro(C, R):
+    d = np.linalg.norm(np.asarray(C, np.float64) - R)
+    r = np.linalg.norm(R)
+    return d / r if r > 0 else d
+
+
+def dequant(Bq, sc, dtype):
+    h = Bq.cpu().numpy()
+    if dtype == cola.F32:
+        return h
+    if dtype == cola.BF16:
+        return (h.astype(np.uint32) << 16).view(np.float32)
+    if dtype == cola.F16:
+        return h.view(np.float16).astype(np.float32)
+    return orc.fp8_to_f32(h) * sc.cpu().numpy()[:, None]
+
+
+def make_problems(k, dtype, seed=0, log_lo=9, log_hi=13):
+    """Whole-graph SpMMs drawn like the batched configuration (node count
+    log-uniform, degree uniform in [4, 64], dim from {32, 64, 128, 256}),
+    at smaller sizes."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(k):
+        log_n = int(rng.integers(log_lo, log_hi + 1))
+        deg = float(rng.uniform(4, 64))
+        dim = int(rng.choice([32, 64, 128, 256]))
+        g = cola.Graph.rmat(log_n, deg, seed=1000 + i + seed * 97)
+        part = g.partition(1, 0)
+        plan = cola.Plan(part, hub_threshold=256, chunk_nnz=128)
+        B = torch.rand((g.n, dim), device="cuda", dtype=torch.float32) * 2 - 1
+        Bq, sc = cola.quantize_rows(B, dtype)
+        C = torch.full((g.n, dim), 9.0, device="cuda", dtype=torch.float32)
+        out.append(dict(g=g, plan=plan, B=Bq, scales=sc, C=C, dim=dim, Bf=B))
+    return out
+
+
+def individual(pr, dtype, flags=cola.OVERWRITE):
+    C = torch.full_like(pr["C"], 9.0)
+    pr["plan"].spmm(C, pr["B"], None, dim=pr["dim"], dtype=dtype, flags=flags, scales_local=pr["scales"])
+    return C
+
+
+@pytest.mark.parametrize("dtype", [cola.F32, cola.BF16, cola.FP8_E4M3])
+def test_batch_matches_individual_launches_and_oracle(dtype):
+    probs = make_problems(10, dtype, seed=dtype)
+    batch = cola.Batch(probs, dtype=dtype, flags=cola.OVERWRITE)
+    batch.run()
+    torch.cuda.synchronize()
+    for pr in probs:
+        Cb = pr["C"].cpu().numpy()
+        Ci = individual(pr, dtype).cpu().numpy()
+        assert rel_fro(Cb, Ci.astype(np.float64)) <= 1e-6
+        rp, ci, va = pr["g"].arrays()
+        ref = orc.spmm_f64(rp, ci, va, 0, pr["g"].n, dequant(pr["B"], pr["scales"], dtype))
+        assert rel_fro(Cb, ref) <= 1e-5
+
+
+def test_result_is_the_same_alone_and_in_the_mix():
+    """A problem's result does not depend on the other problems in the launch."""
+    probs = make_problems(8, cola.BF16, seed=3)
+    alone = cola.Batch([probs[2]], dtype=cola.BF16)
+    alone.run()
+    torch.cuda.synchronize()
+    solo = probs[2]["C"].clone()
+    mix = cola.Batch(probs, dtype=cola.BF16)
+    probs[2]["C"].fill_(-1.0)
+    mix.run()
+    torch.cuda.synchronize()
+    assert torch.equal(solo, probs[2]["C"])
+    mix2 = cola.Batch(probs[::-1], dtype=cola.BF16)
+    mix2.run()
+    torch.cuda.synchronize()
+    assert torch.equal(solo, probs[2]["C"])
+
+
+def test_graph_capture_and_replay():
+    """Captured batch replayed several times with eager work in between; every
+    replay equals the eager result and follows changes to B."""
+    probs = make_problems(6, cola.BF16, seed=5)
+    batch = cola.Batch(probs, dtype=cola.BF16)
+    batch.run()
+    torch.cuda.synchronize()
+    eager = [p["C"].clone() for p in probs]
+    s = torch.cuda.Stream()
+    batch.capture(s)
+    for rep in range(4):
+        for p in probs:
+            p["C"].fill_(float(rep))
+        torch.cuda.synchronize()
+        batch.replay(s)
+        s.synchronize()
+        for p, e in zip(probs, eager):
+            assert torch.equal(p["C"], e), rep
+        # eager work between replays
+        x = torch.randn(1 << 20, device="cuda")
+        (x * 2).sum().item()
+    # the graph reads B at replay time
+    probs[0]["B"].copy_(cola.quantize_rows(-probs[0]["Bf"], cola.BF16)[0])
+    torch.cuda.synchronize()
+    batch.replay(s)
+    s.synchronize()
+    assert torch.allclose(probs[0]["C"], -eager[0], rtol=0, atol=1e-6)
+
+
+def test_batch_autotune_keeps_results():
+    probs = make_problems(8, cola.BF16, seed=7)
+    batch = cola.Batch(probs, dtype=cola.BF16)
+    batch.run()
+    torch.cuda.synchronize()
+    ref = [p["C"].clone() for p in probs]
+    ms = batch.autotune(iters=2)
+    assert ms > 0
+    for p in probs:
+        p["C"].fill_(5.0)
+    batch.run()
+    torch.cuda.synchronize()
+    for p, r in zip(probs, ref):
+        assert rel_fro(p["C"].cpu().numpy(), r.cpu().numpy().astype(np.float64)) <= 1e-6
+
+
+def test_batch_accumulate_and_partitions_with_remote_rows():
+    g = cola.Graph.rmat(12, 12, seed=11, reorder=True)
+    rp, ci, va = g.arrays()
+    nG, dim = 4, 64
+    P = -(-g.n // nG)
+    B = torch.rand((P * nG, dim), device="cuda") * 2 - 1
+    Bh = B.cpu().numpy()
+    probs = []
+    for rank in range(nG):
+        part = g.partition(nG, rank)
+        plan = cola.Plan(part)
+        packed = torch.empty((max(plan.U_r, 1), dim), device="cuda")
+        plan.pack(B, packed)
+        C = torch.full((P, dim), 1.5, device="cuda")
+        probs.append(dict(plan=plan, B=B[rank * P:(rank + 1) * P], packed=packed, C=C, dim=dim))
+    batch = cola.Batch(probs, dtype=cola.F32, flags=cola.ACCUMULATE)
+    batch.run()
+    torch.cuda.synchronize()
+    for rank, pr in enumerate(probs):
+        M = min(g.n, (rank + 1) * P) - rank * P
+        ref = orc.spmm_f64(rp, ci, va, rank * P, rank * P + M, Bh[:g.n])
+        assert rel_fro(pr["C"].cpu().numpy()[:M] - 1.5, ref) <= 2e-5
+
+
+def test_batch_rejects_bad_dims():
+    probs = make_problems(1, cola.BF16, seed=1)
+    probs[0]["dim"] = 36  # not a multiple of 8 bf16 elements
+    with pytest.raises(cola.ColaError):
+        cola.Batch(probs, dtype=cola.BF16)
