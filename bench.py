@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -27,6 +29,93 @@ METRIC = "SpMM HBM GB/s (% of peak) FP32 vs BF16 at stated error; served req/s +
 
 def bytes_alg(M, nnz, U, N, s_B, s_sc=0):
     return 4 * (M + 1) + nnz * 8 + U * (N * s_B + s_sc) + M * N * 4
+
+
+def batched_section(cola, torch, log):
+    """64 independent SpMMs (node count log-uniform in [2^12, 2^18], degree
+    uniform in [4, 64], dim from {32, 64, 128, 256}, B in BF16) in one
+    persistent launch replayed from a hipGraph, against one launch per SpMM."""
+    from cola_amd import workloads
+    t0 = time.time()
+    specs = workloads.batched_specs(64, seed=7)
+    probs = workloads.build_problems(specs, dtype=cola.BF16)
+    balg = sum(workloads.bytes_alg(p, 2) for p in probs)
+    nnz = sum(p["nnz"] for p in probs)
+    log(f"batched: 64 problems, {nnz} entries, built in {time.time() - t0:.1f}s")
+
+    def individual():
+        for p in probs:
+            p["plan"].spmm(p["C"], p["B"], None, dim=p["dim"], dtype=cola.BF16, flags=cola.OVERWRITE)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record()
+        for _ in range(reps):
+            fn()
+        s1.record()
+        torch.cuda.synchronize()
+        return s0.elapsed_time(s1) / reps
+
+    ms_ind = timed(individual, 5)
+    batch = cola.Batch(probs, dtype=cola.BF16, flags=cola.OVERWRITE)
+    ms_untuned = timed(batch.run, 5)
+    batch.autotune(iters=2)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        batch.capture(stream)
+        ms_graph = timed(lambda: batch.replay(stream), 10)
+    torch.cuda.synchronize()
+    tunings = {}
+    for k, p in enumerate(probs):
+        tunings.setdefault(p["dim"], batch.tuning(k))
+    out = {"problems": 64, "entries": nnz, "dtype": "bf16", "bytes_alg": balg,
+           "ms_one_launch_per_problem": ms_ind, "ms_persistent_untuned": ms_untuned,
+           "ms_persistent_autotuned_graph": ms_graph,
+           "GBps_alg_persistent": balg / (ms_graph * 1e-3) / 1e9,
+           "speedup_vs_per_problem_launches": ms_ind / ms_graph,
+           "tuning_per_dim": {str(d): t for d, t in tunings.items()}}
+    log(f"batched: per-problem launches {ms_ind:.3f} ms, persistent {ms_untuned:.3f} ms, "
+        f"autotuned + graph {ms_graph:.3f} ms ({out['GBps_alg_persistent']:.0f} GB/s alg)")
+    del batch, probs
+    return out, tunings
+
+
+def serving_section(cola, torch, log, args, dim_tunings):
+    """Poisson arrivals over resident graphs: continuous batching (BF16) vs
+    plain per-request FP32 launches, same trace."""
+    from cola_amd import workloads
+    t0 = time.time()
+    specs = workloads.batched_specs(args.serve_graphs, seed=11)
+    res = workloads.build_problems(specs, dtype=cola.BF16, keep_f32=True)
+    for p in res:
+        del p["C"]
+    resident_bytes = sum(p["plan"].info["device_bytes"] + p["B"].numel() * 2 + p["Bf"].numel() * 4 for p in res)
+    log(f"serving: {len(res)} resident graphs ({resident_bytes / 1e9:.1f} GB incl. FP32 copies), "
+        f"built in {time.time() - t0:.1f}s")
+    arr, rid = workloads.poisson_trace(args.serve_rate, args.serve_seconds, len(res), seed=5)
+    budget = 32 << 30
+    out = {"resident_graphs": len(res), "rate_req_per_s": args.serve_rate, "trace_seconds": args.serve_seconds,
+           "memory_budget_bytes": budget}
+    sv = cola.Server(budget, dtype=cola.BF16, max_batch=64, max_inflight=2, kahan=True)
+    ids = [sv.add_resident(p["plan"], p["B"]) for p in res]
+    for d, t in dim_tunings.items():
+        sv.set_tuning(d, col_tile=t["col_tile"], rpw_short=t["rpw_short"], rpw_long=t["rpw_long"])
+    sv.run_trace(arr[:50] - arr[0], rid[:50])  # warm-up
+    recs, wall, peak = sv.run_trace(arr, np.asarray(ids, np.int32)[rid])
+    out["continuous_batching_bf16"] = dict(workloads.latency_stats(recs, wall), peak_pool_bytes=peak)
+    del sv
+    pl = cola.Server(budget, dtype=cola.F32, plain=True)
+    ids = [pl.add_resident(p["plan"], p["Bf"]) for p in res]
+    pl.run_trace(arr[:50] - arr[0], rid[:50])
+    recs, wall, peak = pl.run_trace(arr, np.asarray(ids, np.int32)[rid])
+    out["plain_per_request_f32"] = dict(workloads.latency_stats(recs, wall), peak_pool_bytes=peak)
+    b, p = out["continuous_batching_bf16"], out["plain_per_request_f32"]
+    log(f"serving: batched bf16 {b['req_per_s']:.0f} req/s p50 {b['p50_ms']:.2f} ms p99 {b['p99_ms']:.2f} ms "
+        f"(mean batch {b['mean_batch']:.1f}); plain f32 {p['req_per_s']:.0f} req/s p50 {p['p50_ms']:.2f} ms "
+        f"p99 {p['p99_ms']:.2f} ms")
+    return out
 
 
 def main():
@@ -42,6 +131,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-SpMM persistent-launch section")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
+    ap.add_argument("--serve-graphs", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--tuning", default="auto",
                     help='"auto" (autotune before timing) or a JSON object of plan tuning fields')
     ap.add_argument("--save-tuning", default=None, help="write the FP32 tuning used to this JSON file")
@@ -220,6 +314,12 @@ def main():
                                               "GBps_alg": balg / (ms_ref * 1e-3) / 1e9}
         log(f"reference-semantics path (atomics) {ms_ref:.3f} ms")
         del Cr, zl, zr
+
+    dim_tunings = {}
+    if not args.no_batched:
+        extras["batched"], dim_tunings = batched_section(cola, torch, log)
+    if not args.no_serve:
+        extras["serving"] = serving_section(cola, torch, log, args, dim_tunings)
 
     cpu = None
     if not args.no_cpu:

@@ -9,7 +9,7 @@ FP32 / BF16 / FP16 / FP8-e4m3 storage of B).
 from ._lib import (ACCUMULATE, BF16, F16, F32, FP8_E4M3, KAHAN, OVERWRITE, ColaError,  # noqa: F401
                    LIB_PATH, lib)
 from .partition import Graph, Partition, bin2vec, nnz_split, serialize  # noqa: F401
-from .spmm import (DTYPE_BYTES, DTYPE_NAMES, Batch, Plan, d_csc_zip, d_csr_zip, dist_spmm_ppl_batch,  # noqa: F401
+from .spmm import (DTYPE_BYTES, DTYPE_NAMES, REQUEST_RECORD, Batch, Plan, Server, d_csc_zip, d_csr_zip, dist_spmm_ppl_batch,  # noqa: F401
                    error_norms, pack_remote, ppl0_batch_SpMM, quantize_rows)
 
 __version__ = "0.1.0"

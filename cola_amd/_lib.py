@@ -70,6 +70,11 @@ class BatchTuning(ctypes.Structure):
     _fields_ = [("col_tile", c_i32), ("rpw_short", c_i32), ("rpw_long", c_i32)]
 
 
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("memory_budget", c_i64), ("max_batch", c_i32), ("max_inflight", c_i32), ("dtype", c_i32),
+                ("flags", c_i32), ("plain", c_i32)]
+
+
 class PlanInfo(ctypes.Structure):
     _fields_ = [("num_rows", c_i64), ("num_items", c_i64), ("num_hub_rows", c_i64),
                 ("num_hub_chunks", c_i64), ("nnz", c_i64), ("U_r", c_i64), ("U_l", c_i64),
@@ -120,6 +125,11 @@ SIGNATURES = {
     "cola_batch_capture": [c_vp, c_vp],
     "cola_batch_replay": [c_vp, c_vp],
     "cola_batch_destroy": [c_vp],
+    "cola_server_create": [ctypes.POINTER(ServerOpts), ctypes.POINTER(c_vp)],
+    "cola_server_add_resident": [c_vp, c_vp, c_vp, c_i64, c_vp, c_int, ctypes.POINTER(c_int)],
+    "cola_server_set_tuning": [c_vp, c_int, ctypes.POINTER(BatchTuning)],
+    "cola_server_run_trace": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_dbl), P_i64],
+    "cola_server_destroy": [c_vp],
     "cola_quantize_rows": [c_vp, c_i64, c_i64, c_int, c_int, c_vp, c_i64, c_vp, c_vp],
     "cola_error_norms": [c_vp, c_i64, c_vp, c_i64, c_i64, c_int, c_vp, c_vp],
 }

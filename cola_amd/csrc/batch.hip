@@ -15,9 +15,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <deque>
+#include <map>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "device.hpp"
@@ -267,77 +271,108 @@ static void drop_graph(cola_batch_s *b) {
   b->graph = nullptr;
 }
 
+// Appends one problem's segments (degree bins cut into tasks) and hub table
+// entry; task / hub-thread counters run across the whole work list.
+static cola_status append_problem(const cola_batch_problem &pr, const cola_batch_tuning &t, float *partials,
+                                  int dtype, int flags, std::vector<BatchSeg> *segs, std::vector<HubSeg> *hubs,
+                                  int64_t *task, int64_t *hthreads) {
+  const cola_plan_s *p = pr.plan;
+  const int ev = vec_elems(dtype);
+  int tile = pr.dim;
+  if (t.col_tile > 0 && t.col_tile < pr.dim) tile = std::max(ev, t.col_tile / ev * ev);
+  const int chunks = (tile + ev - 1) / ev;
+  const int G = 1 << ilog2(std::max(chunks, 2));
+  if (G > 64) return fail(COLA_ERR_UNSUPPORTED, "batch: a column tile wider than 64 lane vectors");
+  const int S = 64 / G;
+  const int n_tiles = (pr.dim + tile - 1) / tile;
+  auto clamp = [&](int req) {
+    int r = 1;
+    while (r * 2 <= req && r * 2 <= S) r *= 2;
+    return r;
+  };
+  const PlanBin *bins[3] = {&p->longb, &p->shortb, (flags & COLA_OVERWRITE) ? &p->emptyb : nullptr};
+  const int rpws[3] = {clamp(t.rpw_long > 0 ? t.rpw_long : 1), clamp(t.rpw_short > 0 ? t.rpw_short : S), S};
+  for (int bi = 0; bi < 3; ++bi) {
+    const PlanBin *bin = bins[bi];
+    if (!bin || bin->n == 0) continue;
+    BatchSeg s{};
+    s.items = bin->items;
+    s.cols = p->cols;
+    s.vals = p->vals;
+    s.B = pr.B;
+    s.ldB = pr.ld_B;
+    s.sl = pr.scales;
+    s.Bp = pr.packed;
+    s.ldP = pr.ld_packed;
+    s.sp = pr.packed_scales;
+    s.C = pr.C;
+    s.ldC = pr.ld_C;
+    s.partials = partials;
+    s.pstride = p->pstride;
+    s.n_items = (int32_t)bin->n;
+    s.dim = pr.dim;
+    s.rpw = rpws[bi];
+    s.n_tiles = n_tiles;
+    s.tile_w = tile;
+    s.g_log2 = ilog2(G);
+    const int64_t tasks = (bin->n + s.rpw - 1) / s.rpw * n_tiles;
+    s.task_begin = (int32_t)*task;
+    *task += tasks;
+    if (*task > INT32_MAX) return fail(COLA_ERR_INVALID, "batch: too many tasks");
+    s.task_end = (int32_t)*task;
+    segs->push_back(s);
+  }
+  if (p->num_hubs > 0) {
+    HubSeg h{};
+    h.hub_rows = p->hub_rows;
+    h.hub_ptr = p->hub_ptr;
+    h.partials = partials;
+    h.pstride = p->pstride;
+    h.C = pr.C;
+    h.ldC = pr.ld_C;
+    h.n_hubs = (int32_t)p->num_hubs;
+    h.dim = pr.dim;
+    h.thread_begin = *hthreads;
+    *hthreads += (int64_t)p->num_hubs * pr.dim;
+    hubs->push_back(h);
+  }
+  return COLA_OK;
+}
+
+// Launches the persistent row kernel over a work list and the hub reduction.
+static cola_status launch_tables(const BatchSeg *d_segs, int n_segs, int total_tasks, const HubSeg *d_hubs,
+                                 int n_hubs, int64_t hub_threads, int dtype, int flags, int blocks_per_cu,
+                                 hipStream_t s) {
+  const int accumulate = (flags & COLA_OVERWRITE) ? 0 : 1;
+  const bool kahan = (flags & COLA_KAHAN) != 0;
+  if (total_tasks > 0) {
+    int64_t blocks = std::min<int64_t>(((int64_t)total_tasks + 3) / 4, (int64_t)device_cu_count() * blocks_per_cu);
+    blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
+    switch (dtype) {
+      case COLA_F32: launch_batch<COLA_F32, 4>(d_segs, n_segs, total_tasks, accumulate, kahan, (int)blocks, s); break;
+      case COLA_BF16: launch_batch<COLA_BF16, 8>(d_segs, n_segs, total_tasks, accumulate, kahan, (int)blocks, s); break;
+      case COLA_F16: launch_batch<COLA_F16, 8>(d_segs, n_segs, total_tasks, accumulate, kahan, (int)blocks, s); break;
+      case COLA_FP8_E4M3: launch_batch<COLA_FP8_E4M3, 16>(d_segs, n_segs, total_tasks, accumulate, kahan, (int)blocks, s); break;
+    }
+  }
+  if (hub_threads > 0) {
+    const int64_t blocks = (hub_threads + 255) / 256;
+    hipLaunchKernelGGL(batch_hub_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, d_hubs, n_hubs, hub_threads,
+                       accumulate, kahan ? 1 : 0);
+  }
+  return hip_check(hipGetLastError(), "batch launch");
+}
+
 // Rebuilds the segment / hub tables from the problems and their tunings.
 static cola_status build_tables(cola_batch_s *b) {
   drop_graph(b);
   std::vector<BatchSeg> segs;
   std::vector<HubSeg> hubs;
-  const size_t eb = dtype_bytes(b->dtype);
-  const int ev = vec_elems(b->dtype);
   int64_t task = 0, hthreads = 0;
   for (size_t k = 0; k < b->probs.size(); ++k) {
-    const cola_batch_problem &pr = b->probs[k];
-    const cola_plan_s *p = pr.plan;
-    const cola_batch_tuning &t = b->tune[k];
-    int tile = pr.dim;
-    if (t.col_tile > 0 && t.col_tile < pr.dim) tile = std::max(ev, t.col_tile / ev * ev);
-    const int chunks = (tile + ev - 1) / ev;
-    const int G = 1 << ilog2(std::max(chunks, 2));
-    if (G > 64) return fail(COLA_ERR_UNSUPPORTED, "batch: a column tile wider than 64 lane vectors");
-    const int S = 64 / G;
-    const int n_tiles = (pr.dim + tile - 1) / tile;
-    auto clamp = [&](int req) {
-      int r = 1;
-      while (r * 2 <= req && r * 2 <= S) r *= 2;
-      return r;
-    };
-    const PlanBin *bins[3] = {&p->longb, &p->shortb, (b->flags & COLA_OVERWRITE) ? &p->emptyb : nullptr};
-    const int rpws[3] = {clamp(t.rpw_long > 0 ? t.rpw_long : 1), clamp(t.rpw_short > 0 ? t.rpw_short : S), S};
-    for (int bi = 0; bi < 3; ++bi) {
-      const PlanBin *bin = bins[bi];
-      if (!bin || bin->n == 0) continue;
-      BatchSeg s{};
-      s.items = bin->items;
-      s.cols = p->cols;
-      s.vals = p->vals;
-      s.B = pr.B;
-      s.ldB = pr.ld_B;
-      s.sl = pr.scales;
-      s.Bp = pr.packed;
-      s.ldP = pr.ld_packed;
-      s.sp = pr.packed_scales;
-      s.C = pr.C;
-      s.ldC = pr.ld_C;
-      s.partials = b->partials[k];
-      s.pstride = p->pstride;
-      s.n_items = (int32_t)bin->n;
-      s.dim = pr.dim;
-      s.rpw = rpws[bi];
-      s.n_tiles = n_tiles;
-      s.tile_w = tile;
-      s.g_log2 = ilog2(G);
-      const int64_t tasks = (bin->n + s.rpw - 1) / s.rpw * n_tiles;
-      s.task_begin = (int32_t)task;
-      task += tasks;
-      if (task > INT32_MAX) return fail(COLA_ERR_INVALID, "batch: too many tasks");
-      s.task_end = (int32_t)task;
-      segs.push_back(s);
-    }
-    if (p->num_hubs > 0) {
-      HubSeg h{};
-      h.hub_rows = p->hub_rows;
-      h.hub_ptr = p->hub_ptr;
-      h.partials = b->partials[k];
-      h.pstride = p->pstride;
-      h.C = pr.C;
-      h.ldC = pr.ld_C;
-      h.n_hubs = (int32_t)p->num_hubs;
-      h.dim = pr.dim;
-      h.thread_begin = hthreads;
-      hthreads += (int64_t)p->num_hubs * pr.dim;
-      hubs.push_back(h);
-    }
-    (void)eb;
+    cola_status st = append_problem(b->probs[k], b->tune[k], b->partials[k], b->dtype, b->flags, &segs, &hubs,
+                                    &task, &hthreads);
+    if (st) return st;
   }
   if ((int)segs.size() > b->seg_cap) {
     (void)hipFree(b->d_segs);
@@ -374,27 +409,8 @@ static cola_status build_tables(cola_batch_s *b) {
 }
 
 static cola_status run_batch(cola_batch_s *b, hipStream_t s) {
-  const int accumulate = (b->flags & COLA_OVERWRITE) ? 0 : 1;
-  const bool kahan = (b->flags & COLA_KAHAN) != 0;
-  if (b->total_tasks > 0) {
-    const int64_t waves = b->total_tasks;
-    int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * b->blocks_per_cu);
-    blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
-    const int ev = vec_elems(b->dtype);
-    switch (b->dtype) {
-      case COLA_F32: launch_batch<COLA_F32, 4>(b->d_segs, b->n_segs, b->total_tasks, accumulate, kahan, (int)blocks, s); break;
-      case COLA_BF16: launch_batch<COLA_BF16, 8>(b->d_segs, b->n_segs, b->total_tasks, accumulate, kahan, (int)blocks, s); break;
-      case COLA_F16: launch_batch<COLA_F16, 8>(b->d_segs, b->n_segs, b->total_tasks, accumulate, kahan, (int)blocks, s); break;
-      case COLA_FP8_E4M3: launch_batch<COLA_FP8_E4M3, 16>(b->d_segs, b->n_segs, b->total_tasks, accumulate, kahan, (int)blocks, s); break;
-    }
-    (void)ev;
-  }
-  if (b->hub_threads > 0) {
-    const int64_t blocks = (b->hub_threads + 255) / 256;
-    hipLaunchKernelGGL(batch_hub_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, b->d_hubs, b->n_hubs,
-                       b->hub_threads, accumulate, kahan ? 1 : 0);
-  }
-  return hip_check(hipGetLastError(), "batch launch");
+  return launch_tables(b->d_segs, b->n_segs, b->total_tasks, b->d_hubs, b->n_hubs, b->hub_threads, b->dtype, b->flags,
+                       b->blocks_per_cu, s);
 }
 
 static void free_batch(cola_batch_s *b) {
@@ -622,6 +638,335 @@ cola_status cola_batch_autotune(cola_batch b, int iters, void *stream, float *be
 
 cola_status cola_batch_destroy(cola_batch b) {
   free_batch(b);
+  return COLA_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving: resident graphs + feature matrices, requests admitted against a
+// device memory budget, continuous batching into the persistent kernel with
+// up to max_inflight launches in flight (one stream each).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct LaunchSlot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  bool own_stream = false;
+  bool busy = false;
+  BatchSeg *h_segs = nullptr, *d_segs = nullptr;
+  HubSeg *h_hubs = nullptr, *d_hubs = nullptr;
+  int cap_segs = 0, cap_hubs = 0;
+  std::vector<int64_t> reqs;
+};
+
+struct Resident {
+  cola_batch_problem prob;  // C / ld_C filled per request
+  int64_t out_bytes = 0, hub_bytes = 0;
+};
+
+struct ReqState {
+  int64_t pool_off = -1, pool_bytes = 0;
+  float *C = nullptr;
+  float *partials = nullptr;
+};
+
+double now_us() {
+  return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct cola_server_s {
+  cola_server_opts o{};
+  std::vector<Resident> res;
+  std::map<int, cola_batch_tuning> tune;
+  char *pool = nullptr;
+  int64_t pool_size = 0;
+  std::map<int64_t, int64_t> free_blocks;  // offset -> size
+  std::vector<LaunchSlot> slots;
+  int64_t peak_used = 0, used = 0;
+};
+
+static int64_t pool_alloc(cola_server_s *sv, int64_t bytes) {
+  bytes = (bytes + 255) / 256 * 256;
+  for (auto it = sv->free_blocks.begin(); it != sv->free_blocks.end(); ++it) {
+    if (it->second >= bytes) {
+      const int64_t off = it->first, rest = it->second - bytes;
+      sv->free_blocks.erase(it);
+      if (rest > 0) sv->free_blocks[off + bytes] = rest;
+      sv->used += bytes;
+      sv->peak_used = std::max(sv->peak_used, sv->used);
+      return off;
+    }
+  }
+  return -1;
+}
+
+static void pool_free(cola_server_s *sv, int64_t off, int64_t bytes) {
+  bytes = (bytes + 255) / 256 * 256;
+  sv->used -= bytes;
+  auto it = sv->free_blocks.emplace(off, bytes).first;
+  auto nx = std::next(it);
+  if (nx != sv->free_blocks.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    sv->free_blocks.erase(nx);
+  }
+  if (it != sv->free_blocks.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      sv->free_blocks.erase(it);
+    }
+  }
+}
+
+static void free_server(cola_server_s *sv) {
+  if (!sv) return;
+  for (auto &sl : sv->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.own_stream && sl.stream) (void)hipStreamDestroy(sl.stream);
+    if (sl.h_segs) (void)hipHostFree(sl.h_segs);
+    if (sl.h_hubs) (void)hipHostFree(sl.h_hubs);
+    (void)hipFree(sl.d_segs);
+    (void)hipFree(sl.d_hubs);
+  }
+  (void)hipFree(sv->pool);
+  delete sv;
+}
+
+extern "C" {
+
+cola_status cola_server_create(const cola_server_opts *opts, cola_server *out) {
+  if (!opts || !out) return fail(COLA_ERR_INVALID, "server_create: NULL argument");
+  *out = nullptr;
+  cola_server_opts o = *opts;
+  if (!dtype_bytes(o.dtype)) return fail(COLA_ERR_UNSUPPORTED, "server_create: unknown dtype");
+  if (o.memory_budget <= 0) return fail(COLA_ERR_INVALID, "server_create: memory_budget must be positive");
+  if (o.max_batch <= 0) o.max_batch = 64;
+  if (o.max_inflight <= 0) o.max_inflight = 2;
+  if (o.max_inflight > 16 || o.max_batch > 4096) return fail(COLA_ERR_INVALID, "server_create: limits too large");
+  if (o.flags & ~COLA_KAHAN) return fail(COLA_ERR_INVALID, "server_create: unknown flag");
+  auto *sv = new (std::nothrow) cola_server_s;
+  if (!sv) return fail(COLA_ERR_NOMEM, "server_create: out of host memory");
+  sv->o = o;
+  sv->pool_size = (o.memory_budget + 255) / 256 * 256;
+  cola_status st = hip_check(hipMalloc((void **)&sv->pool, sv->pool_size), "hipMalloc request pool");
+  if (st) {
+    free_server(sv);
+    return st;
+  }
+  sv->free_blocks[0] = sv->pool_size;
+  // Plain mode: per-request launches queued on one stream (ring of 64
+  // contexts); batched mode: one context per stream.
+  const int nslots = o.plain ? 64 : o.max_inflight;
+  const int per_launch = o.plain ? 1 : o.max_batch;
+  sv->slots.resize(nslots);
+  hipStream_t shared = nullptr;
+  for (int i = 0; i < nslots && !st; ++i) {
+    LaunchSlot &sl = sv->slots[i];
+    if (!o.plain || i == 0) {
+      st = hip_check(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking), "hipStreamCreate");
+      sl.own_stream = true;
+      shared = sl.stream;
+    } else {
+      sl.stream = shared;
+    }
+    if (!st) st = hip_check(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming), "hipEventCreate");
+    sl.cap_segs = 3 * per_launch;
+    sl.cap_hubs = per_launch;
+    if (!st) st = hip_check(hipHostMalloc((void **)&sl.h_segs, sl.cap_segs * sizeof(BatchSeg)), "hipHostMalloc");
+    if (!st) st = hip_check(hipHostMalloc((void **)&sl.h_hubs, sl.cap_hubs * sizeof(HubSeg)), "hipHostMalloc");
+    if (!st) st = hip_check(hipMalloc((void **)&sl.d_segs, sl.cap_segs * sizeof(BatchSeg)), "hipMalloc");
+    if (!st) st = hip_check(hipMalloc((void **)&sl.d_hubs, sl.cap_hubs * sizeof(HubSeg)), "hipMalloc");
+  }
+  if (st) {
+    free_server(sv);
+    return st;
+  }
+  *out = sv;
+  return COLA_OK;
+}
+
+cola_status cola_server_add_resident(cola_server sv, cola_plan plan, const void *B, int64_t ld_B,
+                                     const float *scales, int dim, int *id) {
+  if (!sv || !plan || !id) return fail(COLA_ERR_INVALID, "server_add_resident: NULL argument");
+  const int ev = vec_elems(sv->o.dtype);
+  if (dim < 1 || dim % ev || dim / ev > 64 || ld_B < dim || ld_B % ev)
+    return fail(COLA_ERR_UNSUPPORTED, "server_add_resident: dim must be a multiple of the vector width, <= 64 vectors");
+  if (plan->U_r > 0) return fail(COLA_ERR_INVALID, "server_add_resident: the plan must be a whole graph (no remote rows)");
+  if (plan->U_l > 0 && (!B || !aligned(B, 16))) return fail(COLA_ERR_INVALID, "server_add_resident: B not 16-byte aligned");
+  if (sv->o.dtype == COLA_FP8_E4M3 && plan->U_l > 0 && !scales)
+    return fail(COLA_ERR_INVALID, "server_add_resident: FP8 needs row scales");
+  if (plan->num_chunks > 0 && dim > plan->max_dim) return fail(COLA_ERR_INVALID, "server_add_resident: dim > max_dim");
+  Resident r;
+  r.prob = cola_batch_problem{plan, dim, B, ld_B, scales, nullptr, dim, nullptr, nullptr, dim};
+  r.out_bytes = (int64_t)plan->num_rows * dim * 4;
+  r.hub_bytes = plan->num_chunks * plan->pstride * 4;
+  sv->res.push_back(r);
+  *id = (int)sv->res.size() - 1;
+  return COLA_OK;
+}
+
+cola_status cola_server_set_tuning(cola_server sv, int dim, const cola_batch_tuning *t) {
+  if (!sv || !t) return fail(COLA_ERR_INVALID, "server_set_tuning: NULL argument");
+  sv->tune[dim] = *t;
+  return COLA_OK;
+}
+
+cola_status cola_server_run_trace(cola_server sv, int64_t n, const double *arrival_us, const int32_t *resident,
+                                  float *const *outputs, cola_request_record *records, double *wall_us,
+                                  int64_t *peak_bytes) {
+  if (!sv || n < 0 || (n > 0 && (!arrival_us || !resident || !records)))
+    return fail(COLA_ERR_INVALID, "server_run_trace: NULL argument");
+  for (int64_t i = 0; i < n; ++i) {
+    if (resident[i] < 0 || resident[i] >= (int)sv->res.size())
+      return fail(COLA_ERR_INVALID, "server_run_trace: unknown resident id");
+    if (i && arrival_us[i] < arrival_us[i - 1]) return fail(COLA_ERR_INVALID, "server_run_trace: arrivals not sorted");
+    const Resident &r = sv->res[resident[i]];
+    const int64_t need = (outputs && outputs[i] ? 0 : r.out_bytes) + r.hub_bytes;
+    if (need > sv->pool_size) return fail(COLA_ERR_NOMEM, "server_run_trace: a request exceeds the memory budget");
+  }
+  const int flags = COLA_OVERWRITE | (sv->o.flags & COLA_KAHAN);
+  std::vector<ReqState> rs(n);
+  std::deque<int64_t> pending;
+  int64_t next = 0, done = 0;
+  sv->peak_used = sv->used;
+  size_t ring = 0;
+  cola_status st = COLA_OK;
+  const double t0 = now_us();
+  auto admit = [&](int64_t i) -> bool {
+    const Resident &r = sv->res[resident[i]];
+    ReqState &q = rs[i];
+    const bool own = !(outputs && outputs[i]);
+    q.pool_bytes = (own ? (r.out_bytes + 255) / 256 * 256 : 0) + r.hub_bytes;
+    if (q.pool_bytes > 0) {
+      q.pool_off = pool_alloc(sv, q.pool_bytes);
+      if (q.pool_off < 0) return false;
+    }
+    q.C = own ? reinterpret_cast<float *>(sv->pool + q.pool_off) : outputs[i];
+    q.partials = r.hub_bytes > 0
+                     ? reinterpret_cast<float *>(sv->pool + q.pool_off + (own ? (r.out_bytes + 255) / 256 * 256 : 0))
+                     : nullptr;
+    return true;
+  };
+  auto launch = [&](LaunchSlot &sl, const std::vector<int64_t> &batch, double tnow) -> cola_status {
+    std::vector<BatchSeg> segs;
+    std::vector<HubSeg> hubs;
+    int64_t task = 0, hth = 0;
+    for (int64_t i : batch) {
+      Resident &r = sv->res[resident[i]];
+      cola_batch_problem pr = r.prob;
+      pr.C = rs[i].C;
+      pr.ld_C = pr.dim;
+      auto it = sv->tune.find(pr.dim);
+      const cola_batch_tuning t = it != sv->tune.end() ? it->second : cola_batch_tuning{0, 0, 1};
+      cola_status e = append_problem(pr, t, rs[i].partials, sv->o.dtype, flags, &segs, &hubs, &task, &hth);
+      if (e) return e;
+      records[i].dispatch_us = tnow;
+      records[i].batch_size = (int32_t)batch.size();
+    }
+    if ((int)segs.size() > sl.cap_segs || (int)hubs.size() > sl.cap_hubs)
+      return fail(COLA_ERR_INVALID, "server: work list exceeds the launch context");
+    std::copy(segs.begin(), segs.end(), sl.h_segs);
+    std::copy(hubs.begin(), hubs.end(), sl.h_hubs);
+    cola_status e;
+    if (!segs.empty() && (e = hip_check(hipMemcpyAsync(sl.d_segs, sl.h_segs, segs.size() * sizeof(BatchSeg),
+                                                       hipMemcpyHostToDevice, sl.stream), "hipMemcpyAsync")))
+      return e;
+    if (!hubs.empty() && (e = hip_check(hipMemcpyAsync(sl.d_hubs, sl.h_hubs, hubs.size() * sizeof(HubSeg),
+                                                       hipMemcpyHostToDevice, sl.stream), "hipMemcpyAsync")))
+      return e;
+    if ((e = launch_tables(sl.d_segs, (int)segs.size(), (int)task, sl.d_hubs, (int)hubs.size(), hth, sv->o.dtype,
+                           flags, 8, sl.stream)))
+      return e;
+    if ((e = hip_check(hipEventRecord(sl.done, sl.stream), "hipEventRecord"))) return e;
+    sl.reqs = batch;
+    sl.busy = true;
+    return COLA_OK;
+  };
+  while (done < n && !st) {
+    bool progress = false;
+    const double tnow = now_us() - t0;
+    while (next < n && arrival_us[next] <= tnow) {
+      records[next].arrival_us = arrival_us[next];
+      records[next].resident = resident[next];
+      pending.push_back(next++);
+      progress = true;
+    }
+    for (auto &sl : sv->slots) {
+      if (!sl.busy) continue;
+      const hipError_t q = hipEventQuery(sl.done);
+      if (q == hipErrorNotReady) continue;
+      if (q != hipSuccess) {
+        st = hip_check(q, "hipEventQuery");
+        break;
+      }
+      const double tdone = now_us() - t0;
+      for (int64_t i : sl.reqs) {
+        records[i].done_us = tdone;
+        if (rs[i].pool_bytes > 0) pool_free(sv, rs[i].pool_off, rs[i].pool_bytes);
+        ++done;
+      }
+      sl.busy = false;
+      sl.reqs.clear();
+      progress = true;
+    }
+    if (st) break;
+    if (sv->o.plain) {
+      // Every arrived request gets its own launch, in arrival order.
+      while (!pending.empty()) {
+        LaunchSlot &sl = sv->slots[ring % sv->slots.size()];
+        if (sl.busy) break;
+        const int64_t i = pending.front();
+        if (!admit(i)) break;
+        pending.pop_front();
+        if ((st = launch(sl, {i}, now_us() - t0))) break;
+        ++ring;
+        progress = true;
+      }
+    } else {
+      for (auto &sl : sv->slots) {
+        if (sl.busy || pending.empty()) continue;
+        std::vector<int64_t> batch;
+        while (!pending.empty() && (int)batch.size() < sv->o.max_batch) {
+          if (!admit(pending.front())) break;  // first-come first-served: wait for memory
+          batch.push_back(pending.front());
+          pending.pop_front();
+        }
+        if (batch.empty()) break;
+        if ((st = launch(sl, batch, now_us() - t0))) break;
+        progress = true;
+      }
+    }
+    if (!progress) {
+      if (next < n && pending.empty()) {
+        const double wait = arrival_us[next] - (now_us() - t0);
+        if (wait > 200.0) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait - 100.0)));
+      }
+    }
+  }
+  for (auto &sl : sv->slots)
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+  if (st) {
+    for (int64_t i = 0; i < n; ++i)
+      if (rs[i].pool_off >= 0 && records[i].done_us == 0.0 && rs[i].pool_bytes > 0)
+        pool_free(sv, rs[i].pool_off, rs[i].pool_bytes);
+    for (auto &sl : sv->slots) {
+      sl.busy = false;
+      sl.reqs.clear();
+    }
+    return st;
+  }
+  if (wall_us) *wall_us = now_us() - t0;
+  if (peak_bytes) *peak_bytes = sv->peak_used;
+  return COLA_OK;
+}
+
+cola_status cola_server_destroy(cola_server sv) {
+  free_server(sv);
   return COLA_OK;
 }
 

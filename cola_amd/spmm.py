@@ -267,6 +267,58 @@ class Batch:
         return dict(col_tile=t.col_tile, rpw_short=t.rpw_short, rpw_long=t.rpw_long, blocks_per_cu=bpc.value)
 
 
+REQUEST_RECORD = np.dtype([("arrival_us", "f8"), ("dispatch_us", "f8"), ("done_us", "f8"), ("resident", "i4"),
+                           ("batch_size", "i4")])
+
+
+class Server:
+    """Continuous-batching SpMM server over resident graphs (see
+    cola_server_run_trace in include/cola_spmm.h)."""
+
+    def __init__(self, memory_budget, dtype=BF16, max_batch=64, max_inflight=2, kahan=False, plain=False):
+        o = _lib.ServerOpts(int(memory_budget), int(max_batch), int(max_inflight), int(dtype),
+                            KAHAN if kahan else 0, 1 if plain else 0)
+        h = ctypes.c_void_p()
+        check(lib().cola_server_create(ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self._keep = []
+        self.dtype = dtype
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib._lib is not None:
+            _lib._lib.cola_server_destroy(self._h)
+            self._h = None
+
+    def add_resident(self, plan, B, dim=None, scales=None):
+        rid = ctypes.c_int()
+        dim = dim if dim is not None else B.shape[1]
+        check(lib().cola_server_add_resident(self._h, plan.handle, _dptr(B), B.stride(0), _dptr(scales), int(dim),
+                                             ctypes.byref(rid)))
+        self._keep.append((plan, B, scales))
+        return rid.value
+
+    def set_tuning(self, dim, col_tile=0, rpw_short=0, rpw_long=0):
+        t = _lib.BatchTuning(int(col_tile), int(rpw_short), int(rpw_long))
+        check(lib().cola_server_set_tuning(self._h, int(dim), ctypes.byref(t)))
+
+    def run_trace(self, arrival_us, resident, outputs=None):
+        """Serves the trace in real time; returns (records, wall_us, peak_pool_bytes)."""
+        arr = np.ascontiguousarray(arrival_us, np.float64)
+        res = np.ascontiguousarray(resident, np.int32)
+        n = len(arr)
+        recs = np.zeros(n, REQUEST_RECORD)
+        outp = None
+        if outputs is not None:
+            outp = (ctypes.c_void_p * n)(*[(o.data_ptr() if o is not None else None) for o in outputs])
+        wall = ctypes.c_double()
+        peak = ctypes.c_int64()
+        check(lib().cola_server_run_trace(self._h, n, arr.ctypes.data_as(ctypes.c_void_p),
+                                          res.ctypes.data_as(ctypes.c_void_p), outp,
+                                          recs.ctypes.data_as(ctypes.c_void_p), ctypes.byref(wall),
+                                          ctypes.byref(peak)))
+        return recs, wall.value, peak.value
+
+
 def quantize_rows(src, dtype, dst=None, scales=None, stream=None):
     """FP32 rows -> storage dtype (RNE; FP8 with per-row scale max|x|/448)."""
     torch = _torch()

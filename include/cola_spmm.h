@@ -350,6 +350,53 @@ cola_status cola_batch_capture(cola_batch b, void *stream);
 cola_status cola_batch_replay(cola_batch b, void *stream);
 cola_status cola_batch_destroy(cola_batch b);
 
+/* ------------------------------------------------------------------------ */
+/* Serving: continuous batching of SpMM requests over resident graphs.       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct cola_server_s *cola_server;
+typedef struct {
+  int64_t memory_budget; /* device bytes for request outputs + hub workspace;
+                            a request is admitted when its share fits      */
+  int32_t max_batch;     /* requests per launch (0 = 64)                   */
+  int32_t max_inflight;  /* launches in flight, one stream each (0 = 2)    */
+  int32_t dtype;         /* storage dtype of every resident feature matrix */
+  int32_t flags;         /* COLA_KAHAN or 0 (outputs are overwritten)      */
+  int32_t plain;         /* 1: no batching, every request is its own launch
+                            queued on one stream as it arrives (baseline)  */
+} cola_server_opts;
+typedef struct {
+  double arrival_us, dispatch_us, done_us; /* from the start of the trace  */
+  int32_t resident;                        /* which resident SpMM          */
+  int32_t batch_size;                      /* requests in its launch       */
+} cola_request_record;
+
+cola_status cola_server_create(const cola_server_opts *opts, cola_server *out);
+/* Registers a resident SpMM: a whole-graph plan and its feature matrix B
+ * (device memory, kept by the caller).  Returns its id. */
+cola_status cola_server_add_resident(cola_server s, cola_plan plan,
+                                     const void *B, int64_t ld_B,
+                                     const float *scales, int dim, int *id);
+/* Schedule used for requests of one dim (e.g. from cola_batch_autotune). */
+cola_status cola_server_set_tuning(cola_server s, int dim,
+                                   const cola_batch_tuning *t);
+/* Serves a request trace in real time: request i arrives arrival_us[i]
+ * microseconds after the start (non-decreasing) and asks for C = A·B of
+ * resident[i].  Arrived requests are admitted first-come first-served when
+ * their output and workspace fit in the memory budget, and every free launch
+ * slot takes all admitted requests (up to max_batch) into one persistent
+ * launch, so requests arriving while others run join the next launch and
+ * launches overlap on the device.  outputs[i] (or outputs NULL / entry NULL:
+ * a pool buffer freed at completion) receives request i's C (ld = dim).
+ * Fills records[i]; *wall_us = trace duration; *peak_bytes = pool peak. */
+cola_status cola_server_run_trace(cola_server s, int64_t n,
+                                  const double *arrival_us,
+                                  const int32_t *resident,
+                                  float *const *outputs,
+                                  cola_request_record *records,
+                                  double *wall_us, int64_t *peak_bytes);
+cola_status cola_server_destroy(cola_server s);
+
 /* Row-wise conversion of FP32 rows to a storage dtype (round to nearest
  * even; FP8 rows get scale = max|row|/448 and are stored as x/scale).
  * dst has ld_dst elements per row; scales (FP8 only) has rows entries. */

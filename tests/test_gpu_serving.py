@@ -1,0 +1,94 @@
+"""Serving: continuous batching over resident graphs, admission by memory
+budget, overlap of launches, and results that are the same alone and in the
+mix (bitwise: each item is reduced by one team in a fixed order whatever the
+launch composition)."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import cola_amd as cola  # noqa: E402
+from cola_amd import workloads  # noqa: E402
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a GPU")]
+
+
+@pytest.fixture(scope="module")
+def residents():
+    specs = workloads.batched_specs(12, seed=4, log_lo=9, log_hi=14)
+    return workloads.build_problems(specs, dtype=cola.BF16, plan_opts=dict(hub_threshold=256, chunk_nnz=128))
+
+
+def reference(p):
+    C = torch.zeros_like(p["C"])
+    p["plan"].spmm(C, p["B"], None, dim=p["dim"], dtype=cola.BF16, flags=cola.OVERWRITE)
+    torch.cuda.synchronize()
+    return C
+
+
+def make_server(residents, budget=1 << 30, **kw):
+    sv = cola.Server(budget, dtype=cola.BF16, **kw)
+    ids = [sv.add_resident(p["plan"], p["B"]) for p in residents]
+    return sv, ids
+
+
+def test_same_result_alone_and_in_the_mix(residents):
+    sv, ids = make_server(residents, max_batch=16, max_inflight=2)
+    x = 5
+    alone = torch.full_like(residents[x]["C"], -3.0)
+    recs, _, _ = sv.run_trace([0.0], [ids[x]], outputs=[alone])
+    assert recs["batch_size"][0] == 1
+    assert torch.allclose(alone, reference(residents[x]), rtol=0, atol=1e-5)
+    # A Poisson mix in which resident x appears several times.
+    arr, res = workloads.poisson_trace(4000, 0.05, len(ids), seed=1)
+    res[::7] = ids[x]
+    outs = [torch.full_like(residents[r]["C"], 7.0) if r == ids[x] else None for r in res]
+    recs, wall, peak = sv.run_trace(arr, res, outputs=outs)
+    assert np.all(recs["done_us"] >= recs["arrival_us"])
+    mixed = [o for o in outs if o is not None]
+    assert len(mixed) >= 3
+    assert recs["batch_size"].max() > 1, "requests were never batched together"
+    for o in mixed:
+        assert torch.equal(o, alone)
+
+
+def test_every_request_is_correct_and_launches_overlap(residents):
+    sv, ids = make_server(residents, max_batch=8, max_inflight=3)
+    arr, res = workloads.poisson_trace(6000, 0.05, len(ids), seed=2)
+    outs = [torch.empty_like(residents[r]["C"]) for r in res]
+    recs, wall, peak = sv.run_trace(arr, res, outputs=outs)
+    refs = {r: reference(residents[r]) for r in set(res.tolist())}
+    for o, r in zip(outs, res):
+        assert torch.allclose(o, refs[r], rtol=0, atol=1e-5)
+    # continuous batching: some launch was dispatched while an earlier one was still running
+    order = np.argsort(recs["dispatch_us"])
+    d, e = recs["dispatch_us"][order], recs["done_us"][order]
+    assert np.any(d[1:] < np.maximum.accumulate(e)[:-1]), "no overlap of launches observed"
+
+
+def test_admission_respects_the_memory_budget(residents):
+    # room for the largest single request's output + workspace, not much more
+    need = max(p["n"] * p["dim"] * 4 + p["plan"].info["num_hub_chunks"] * 256 * 4 for p in residents)
+    budget = int(need * 1.5)
+    sv, ids = make_server(residents, budget=budget, max_batch=32, max_inflight=2)
+    arr, res = workloads.poisson_trace(20000, 0.02, len(ids), seed=3)
+    recs, wall, peak = sv.run_trace(arr, res)
+    assert peak <= budget
+    assert np.all(recs["done_us"] > 0)
+
+
+def test_plain_mode_matches(residents):
+    sv, ids = make_server(residents, plain=True)
+    arr, res = workloads.poisson_trace(3000, 0.03, len(ids), seed=4)
+    outs = [torch.empty_like(residents[r]["C"]) for r in res]
+    recs, wall, peak = sv.run_trace(arr, res, outputs=outs)
+    assert np.all(recs["batch_size"] == 1)
+    for o, r in zip(outs, res):
+        assert torch.allclose(o, reference(residents[r]), rtol=0, atol=1e-5)
+
+
+def test_request_larger_than_budget_is_rejected(residents):
+    sv, ids = make_server(residents, budget=1 << 12)
+    with pytest.raises(cola.ColaError):
+        sv.run_trace([0.0], [ids[0]])
