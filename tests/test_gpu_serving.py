@@ -27,6 +27,16 @@ def reference(p):
     return C
 
 
+def bursty_trace(n_ids, rate, seconds, seed, burst=24):
+    """A burst of requests at t = 0 (contention for the launch slots) followed
+    by Poisson arrivals."""
+    arr, res = workloads.poisson_trace(rate, seconds, n_ids, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    arr = np.concatenate([np.zeros(burst), arr])
+    res = np.concatenate([rng.integers(0, n_ids, burst).astype(np.int32), res])
+    return arr, res
+
+
 def make_server(residents, budget=1 << 30, **kw):
     sv = cola.Server(budget, dtype=cola.BF16, **kw)
     ids = [sv.add_resident(p["plan"], p["B"]) for p in residents]
@@ -41,7 +51,7 @@ def test_same_result_alone_and_in_the_mix(residents):
     assert recs["batch_size"][0] == 1
     assert torch.allclose(alone, reference(residents[x]), rtol=0, atol=1e-5)
     # A Poisson mix in which resident x appears several times.
-    arr, res = workloads.poisson_trace(4000, 0.05, len(ids), seed=1)
+    arr, res = bursty_trace(len(ids), 4000, 0.05, seed=1)
     res[::7] = ids[x]
     outs = [torch.full_like(residents[r]["C"], 7.0) if r == ids[x] else None for r in res]
     recs, wall, peak = sv.run_trace(arr, res, outputs=outs)
@@ -55,7 +65,7 @@ def test_same_result_alone_and_in_the_mix(residents):
 
 def test_every_request_is_correct_and_launches_overlap(residents):
     sv, ids = make_server(residents, max_batch=8, max_inflight=3)
-    arr, res = workloads.poisson_trace(6000, 0.05, len(ids), seed=2)
+    arr, res = bursty_trace(len(ids), 6000, 0.05, seed=2)
     outs = [torch.empty_like(residents[r]["C"]) for r in res]
     recs, wall, peak = sv.run_trace(arr, res, outputs=outs)
     refs = {r: reference(residents[r]) for r in set(res.tolist())}
