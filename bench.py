@@ -137,7 +137,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--tuning", default="auto",
-                    help='"auto" (autotune before timing) or a JSON object of plan tuning fields')
+                    help='"auto" (autotune before timing), a JSON object of plan tuning fields, or '
+                         '@file.json (its "tuning" object, e.g. a committed profile)')
     ap.add_argument("--save-tuning", default=None, help="write the FP32 tuning used to this JSON file")
     ap.add_argument("--copy-calibrate-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (counter calibration)")
@@ -190,7 +191,13 @@ def main():
         tuning, tune_ms = plan.autotune(C, B_local, packed, iters=3)
         log(f"autotuned f32: {tuning} ({tune_ms:.3f} ms)")
     else:
-        plan.tuning = json.loads(args.tuning)
+        if args.tuning.startswith("@"):
+            with open(os.path.join(ROOT, args.tuning[1:])) as f:
+                tj = json.load(f)
+            tj = tj.get("tuning", tj)
+        else:
+            tj = json.loads(args.tuning)
+        plan.tuning = tj
         tuning = plan.tuning
         log(f"fixed f32 tuning: {tuning}")
     if args.save_tuning:
