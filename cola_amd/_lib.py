@@ -52,13 +52,13 @@ class PartitionArrays(ctypes.Structure):
 
 class PlanOpts(ctypes.Structure):
     _fields_ = [("hub_threshold", c_i32), ("chunk_nnz", c_i32), ("kahan_degree", c_i32),
-                ("max_dim", c_i32), ("short_degree", c_i32)]
+                ("max_dim", c_i32), ("short_degree", c_i32), ("col_panels", c_i32)]
 
 
 class PlanTuning(ctypes.Structure):
     _fields_ = [("col_tile", c_i32), ("rpw_short", c_i32), ("rpw_long", c_i32), ("blocks_per_cu", c_i32),
                 ("xcd_remap", c_i32), ("split_remote", c_i32),
-                ("unroll", c_i32)]
+                ("unroll", c_i32), ("panels", c_i32)]
 
 
 class BatchProblem(ctypes.Structure):

@@ -63,6 +63,16 @@ struct HostPlan {
   std::vector<int32_t> hub_rows;     // output row of each hub
   std::vector<int32_t> hub_chunk_ptr;// chunk range per hub (num_hubs+1)
   std::vector<int32_t> remote_cols;  // unique remote global columns (U_r)
+  // Column panels (col_panels = K > 1): 2K passes, local columns cut into K
+  // equal ranges then remote packed rows into K; panel k's items are the
+  // runs of each row's entries whose column falls in range k (hub rows cut
+  // into chunks numbered row by row, so a hub's chunks stay contiguous).
+  int32_t panels = 0;
+  std::vector<WorkItem> panel_items;    // all panels back to back
+  std::vector<int64_t> panel_begin;     // 2K+1 offsets into panel_items
+  std::vector<int32_t> panel_xcd;       // 2K x 9 item ranges (relative to the panel)
+  std::vector<int32_t> panel_hub_ptr;   // chunk range per hub (num_hubs+1)
+  int32_t panel_chunks = 0;
 };
 
 cola_status build_plan(const Partition &p, const cola_plan_opts &opts,

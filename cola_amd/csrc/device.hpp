@@ -51,5 +51,11 @@ struct cola_plan_s {
   float *vals = nullptr, *partials = nullptr;
   int64_t pstride = 0;
   int64_t device_bytes = 0;
-  cola_plan_tuning tune{0, 0, 1, 8, 1, 0, 8};
+  // Column-panel schedule (see HostPlan): 2*panels item lists.
+  int32_t panels = 0;
+  int4 *panel_items = nullptr;
+  int32_t *panel_xcd = nullptr, *panel_hub_ptr = nullptr;
+  std::vector<int64_t> panel_begin;
+  int64_t panel_chunks = 0;
+  cola_plan_tuning tune{0, 0, 1, 8, 1, 0, 8, 0};
 };

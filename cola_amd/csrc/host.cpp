@@ -522,6 +522,55 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
   split(out->items_long, &out->xcd_long);
   split(out->items_short_r, &out->xcd_short_r);
   split(out->items_long_r, &out->xcd_long_r);
+
+  // Column panels.
+  const int32_t K = o.col_panels > 1 ? o.col_panels : 0;
+  out->panels = K;
+  out->panel_items.clear();
+  out->panel_begin.assign(1, 0);
+  out->panel_xcd.clear();
+  out->panel_hub_ptr.assign(1, 0);
+  out->panel_chunks = 0;
+  if (K > 1) {
+    const int64_t P = std::max<int32_t>(1, p.nodesPerPE), U = std::max<int32_t>(1, out->U_r);
+    std::vector<std::vector<WorkItem>> per(2 * K);
+    int32_t pch = 0;
+    for (int64_t i = 0; i < M; ++i) {
+      const int64_t b = cnt[i], e = cnt[i + 1], d = e - b;
+      if (d == 0) continue;
+      const bool is_hub = d > hub;
+      const int32_t kflag = d > kdeg ? 1 : 0;
+      int64_t s = b;
+      while (s < e) {
+        const int32_t c = out->cols[s];
+        const int key = c >= 0 ? (int)((int64_t)c * K / P) : K + (int)((int64_t)(~c) * K / U);
+        int64_t t = s + 1;
+        while (t < e) {
+          const int32_t c2 = out->cols[t];
+          const int k2 = c2 >= 0 ? (int)((int64_t)c2 * K / P) : K + (int)((int64_t)(~c2) * K / U);
+          if (k2 != key) break;
+          ++t;
+        }
+        if (!is_hub) {
+          per[key].push_back({(int32_t)i, (int32_t)s, (int32_t)t, kflag});
+        } else {
+          for (int64_t q = s; q < t; q += chunk)
+            per[key].push_back({-(pch++ + 1), (int32_t)q, (int32_t)std::min<int64_t>(t, q + chunk), kflag});
+        }
+        s = t;
+      }
+      if (is_hub) out->panel_hub_ptr.push_back(pch);
+    }
+    out->panel_chunks = pch;
+    for (int k = 0; k < 2 * K; ++k) {
+      std::vector<int32_t> xb;
+      split(per[k], &xb);
+      out->panel_xcd.insert(out->panel_xcd.end(), xb.begin(), xb.end());
+      out->panel_items.insert(out->panel_items.end(), per[k].begin(), per[k].end());
+      out->panel_begin.push_back((int64_t)out->panel_items.size());
+    }
+    if (out->panel_items.size() > (size_t)INT32_MAX) return fail(COLA_ERR_INVALID, "plan: too many panel items");
+  }
   return COLA_OK;
 }
 

@@ -291,6 +291,13 @@ __global__ __launch_bounds__(256) void zero_rows_kernel(const int32_t *__restric
   for (int col = lane; col < dim; col += 64) dst[col] = 0.f;
 }
 
+__global__ __launch_bounds__(256) void zero_matrix_kernel(float *__restrict__ C, int64_t ldC, int64_t rows,
+                                                          int dim) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = rows * dim;
+  for (int64_t i = t; i < total; i += (int64_t)gridDim.x * blockDim.x) C[(i / dim) * ldC + i % dim] = 0.f;
+}
+
 // One wave per row.  FP8 rows: scale = max|x| / 448 (1 for an all-zero row).
 __global__ __launch_bounds__(256) void quantize_rows_kernel(const float *__restrict__ src,
                                                             int64_t lds, int64_t rows, int dim,
@@ -387,6 +394,9 @@ static void free_plan(cola_plan_s *p) {
   (void)hipFree(p->hub_ptr);
   (void)hipFree(p->remote_cols);
   (void)hipFree(p->partials);
+  (void)hipFree(p->panel_items);
+  (void)hipFree(p->panel_xcd);
+  (void)hipFree(p->panel_hub_ptr);
   delete p;
 }
 
@@ -444,7 +454,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
   if (c.flags & ~(COLA_OVERWRITE | COLA_KAHAN)) return fail(COLA_ERR_INVALID, "spmm_partition: unknown flag");
   if (c.dim < 1 || c.ldC < c.dim || c.ldB < c.dim || (p->U_r > 0 && c.ldP < c.dim))
     return fail(COLA_ERR_INVALID, "spmm_partition: bad dim or leading dimension");
-  if (c.dim > p->max_dim && p->num_chunks > 0)
+  if (c.dim > p->max_dim && (p->num_chunks > 0 || p->panel_chunks > 0))
     return fail(COLA_ERR_INVALID, "spmm_partition: dim exceeds the plan's max_dim");
   if (p->num_rows > 0 && !c.C) return fail(COLA_ERR_INVALID, "spmm_partition: C is NULL");
   if (p->U_l > 0 && !c.Bl) return fail(COLA_ERR_INVALID, "spmm_partition: B_local is NULL");
@@ -467,6 +477,61 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
                        (int)p->num_empty, c.C, c.ldC, c.dim);
   }
   const int bpc = t.blocks_per_cu > 0 ? t.blocks_per_cu : 8;
+  if (t.panels && p->panels > 1) {
+    // Column-panel schedule: zero C (overwrite), then one accumulating pass
+    // per (tile, panel), then the hub rows' partial sums added on top.
+    if (!accumulate) {
+      const int64_t total = (int64_t)p->num_rows * c.dim;
+      const int64_t blocks = std::min<int64_t>((total + 255) / 256, (int64_t)device_cu_count() * 64);
+      if (total > 0)
+        hipLaunchKernelGGL(zero_matrix_kernel, dim3((unsigned)std::max<int64_t>(1, blocks)), dim3(256), 0, s, c.C,
+                           c.ldC, (int64_t)p->num_rows, c.dim);
+    }
+    for (int c0 = 0; c0 < c.dim; c0 += tile) {
+      const int w = std::min(tile, c.dim - c0);
+      Shape sh;
+      if (!cola_dev::pick_shape(c.dtype, w, vec_ok, &sh))
+        return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: dim " + std::to_string(c.dim) + " not supported");
+      const int S = 64 / sh.g;
+      for (int k = 0; k < 2 * p->panels; ++k) {
+        const int64_t n = p->panel_begin[k + 1] - p->panel_begin[k];
+        if (n == 0) continue;
+        cola_dev::RowArgs a;
+        a.items = p->panel_items + p->panel_begin[k];
+        a.xcd_begin = p->panel_xcd + 9 * k;
+        a.cols = p->cols;
+        a.vals = p->vals;
+        a.Bl = c.Bl ? (const char *)c.Bl + (size_t)c0 * eb : nullptr;
+        a.ldB = c.ldB;
+        a.sl = c.sl;
+        a.Bp = c.Bp ? (const char *)c.Bp + (size_t)c0 * eb : nullptr;
+        a.ldP = c.ldP;
+        a.sp = c.sp;
+        a.C = c.C + c0;
+        a.ldC = c.ldC;
+        a.partials = p->partials ? p->partials + c0 : nullptr;
+        a.pstride = p->pstride;
+        a.dim = w;
+        a.accumulate = 1;
+        a.kahan = kahan;
+        a.rpw = clamp_rpw(t.rpw_long > 0 ? t.rpw_long : 1, S);
+        a.xcd_remap = t.xcd_remap ? 1 : 0;
+        a.phase = 0;
+        const int64_t waves = (n + a.rpw - 1) / a.rpw;
+        int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bpc);
+        blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
+        if (!launch_rows(c.dtype, sh, a, (int)blocks, s, t.unroll == 16))
+          return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: no kernel for this dim");
+      }
+    }
+    if (p->num_hubs > 0) {
+      const int64_t threads = p->num_hubs * (int64_t)c.dim;
+      const int64_t blocks = (threads + 255) / 256;
+      hipLaunchKernelGGL(hub_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p->hub_rows, p->panel_hub_ptr,
+                         (int)p->num_hubs, p->partials, p->pstride, c.C, c.ldC, c.dim, 1, kahan);
+    }
+    return hip_check(hipGetLastError(), "spmm_partition launch");
+  }
   for (int c0 = 0; c0 < c.dim; c0 += tile) {
     const int w = std::min(tile, c.dim - c0);
     Shape sh;
@@ -754,18 +819,31 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
       return st;
     }
   }
-  if (p->num_chunks > 0) {
-    st = hip_check(hipMalloc((void **)&p->partials, (size_t)p->num_chunks * p->pstride * sizeof(float)),
+  if (hp.panels > 1) {
+    std::vector<int4> pi = to_int4(hp.panel_items);
+    p->panels = hp.panels;
+    p->panel_begin = hp.panel_begin;
+    p->panel_chunks = hp.panel_chunks;
+    if ((st = upload(pi, &p->panel_items)) || (st = upload(hp.panel_xcd, &p->panel_xcd)) ||
+        (st = upload(hp.panel_hub_ptr, &p->panel_hub_ptr))) {
+      free_plan(p);
+      return st;
+    }
+    p->device_bytes += (int64_t)(pi.size() * sizeof(int4) + hp.panel_xcd.size() * 4 + hp.panel_hub_ptr.size() * 4);
+  }
+  const int64_t chunks_alloc = std::max<int64_t>(p->num_chunks, p->panel_chunks);
+  if (chunks_alloc > 0) {
+    st = hip_check(hipMalloc((void **)&p->partials, (size_t)chunks_alloc * p->pstride * sizeof(float)),
                    "hipMalloc partials");
     if (st) {
       free_plan(p);
       return st;
     }
   }
-  p->device_bytes = (int64_t)((is.size() + il.size() + isr.size() + ilr.size()) * sizeof(int4) + 36 * 4 + hp.cols.size() * 4 +
+  p->device_bytes += (int64_t)((is.size() + il.size() + isr.size() + ilr.size()) * sizeof(int4) + 36 * 4 + hp.cols.size() * 4 +
                               hp.vals.size() * 4 + hp.empty_rows.size() * 4 + hp.hub_rows.size() * 4 +
                               hp.hub_chunk_ptr.size() * 4 + hp.remote_cols.size() * 4 +
-                              (size_t)p->num_chunks * p->pstride * 4);
+                              (size_t)chunks_alloc * p->pstride * 4);
   *out = p;
   return COLA_OK;
 }
@@ -789,7 +867,8 @@ cola_status cola_plan_set_tuning(cola_plan p, const cola_plan_tuning *t) {
   if (!p || !t) return fail(COLA_ERR_INVALID, "plan_set_tuning: NULL argument");
   if (t->col_tile < 0 || t->rpw_short < 0 || t->rpw_short > 64 || t->rpw_long < 0 || t->rpw_long > 64 ||
       t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || (t->xcd_remap != 0 && t->xcd_remap != 1) ||
-      (t->split_remote != 0 && t->split_remote != 1) || (t->unroll != 0 && t->unroll != 8 && t->unroll != 16))
+      (t->split_remote != 0 && t->split_remote != 1) || (t->unroll != 0 && t->unroll != 8 && t->unroll != 16) ||
+      (t->panels != 0 && t->panels != 1))
     return fail(COLA_ERR_INVALID, "plan_set_tuning: parameter out of range");
   p->tune = *t;
   return COLA_OK;
@@ -896,13 +975,21 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
       t.unroll = u;
       cands.push_back(t);
     }
+    if (p->panels > 1)
+      for (int x : {0, 1}) {
+        cola_plan_tuning t = best;
+        t.panels = x;
+        cands.push_back(t);
+      }
     for (const auto &t : cands) {
       float ms = 0.f;
       if ((st = time_of(t, &ms))) break;
       if (verbose)
         fprintf(stderr,
-                "[autotune] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d split=%d unroll=%d: %.4f ms\n",
-                t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.xcd_remap, t.split_remote, t.unroll, ms);
+                "[autotune] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d split=%d unroll=%d panels=%d: "
+                "%.4f ms\n",
+                t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.xcd_remap, t.split_remote, t.unroll, t.panels,
+                ms);
       if (ms < bms * 0.99f) {
         bms = ms;
         best = t;

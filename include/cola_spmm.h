@@ -241,6 +241,9 @@ typedef struct {
                             (0 = default 256)                               */
   int32_t short_degree;  /* rows with at most this many entries form the
                             short-row bin (0 = default 32)                  */
+  int32_t col_panels;    /* K > 1: also build a column-panel schedule (local
+                            columns and packed rows each cut into K ranges,
+                            one pass per range) for cache reuse; 0 = none   */
 } cola_plan_opts;
 cola_status cola_plan_create(cola_partition p, const cola_plan_opts *opts,
                              cola_plan *out);
@@ -266,6 +269,9 @@ typedef struct {
                             one pass (default)                              */
   int32_t unroll;        /* row-gather loads in flight per lane group: 8
                             (default, also 0) or 16                         */
+  int32_t panels;        /* 1: run the column-panel schedule (plans built
+                            with col_panels > 1): C is zeroed (overwrite),
+                            then one accumulating pass per panel; 0: off    */
 } cola_plan_tuning;
 cola_status cola_plan_set_tuning(cola_plan plan, const cola_plan_tuning *t);
 cola_status cola_plan_get_tuning(cola_plan plan, cola_plan_tuning *t);

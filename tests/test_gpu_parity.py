@@ -361,12 +361,15 @@ def test_full_size_partition_checksum():
     assert rel_fro(Ch[sample], refs) <= FP32_TOL
 
 
-TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=8),
+TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=8, panels=0),
            dict(col_tile=32, rpw_short=1, rpw_long=2, blocks_per_cu=2, xcd_remap=0, split_remote=1),
            dict(col_tile=16, rpw_short=2, rpw_long=4, blocks_per_cu=16, xcd_remap=1, split_remote=0),
            dict(col_tile=48, rpw_short=64, rpw_long=64, blocks_per_cu=1, xcd_remap=0, split_remote=1),
            dict(col_tile=0, rpw_short=4, rpw_long=1, blocks_per_cu=4, xcd_remap=1, split_remote=1, unroll=16),
-           dict(col_tile=32, rpw_short=8, rpw_long=2, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=16)]
+           dict(col_tile=32, rpw_short=8, rpw_long=2, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=16),
+           dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=8, panels=1),
+           dict(col_tile=32, rpw_short=0, rpw_long=2, blocks_per_cu=4, xcd_remap=0, split_remote=0, unroll=8,
+                panels=1)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
@@ -375,7 +378,7 @@ def test_tunings_give_the_same_result(c1, tuning, dtype):
     """Column tiles, rows per wave, grid size and XCD remap change the schedule,
     never the result: the same FP32-accumulated sums up to rounding order."""
     dim = 64
-    plan = cola.Plan(c1.part, short_degree=24, hub_threshold=256, chunk_nnz=128)
+    plan = cola.Plan(c1.part, short_degree=24, hub_threshold=256, chunk_nnz=128, col_panels=4)
     B = c1.B
     Bq, sc = cola.quantize_rows(B, dtype)
     tdt = {cola.F32: torch.float32, cola.BF16: torch.uint16, cola.FP8_E4M3: torch.uint8}[dtype]
