@@ -52,7 +52,8 @@ class PartitionArrays(ctypes.Structure):
 
 class PlanOpts(ctypes.Structure):
     _fields_ = [("hub_threshold", c_i32), ("chunk_nnz", c_i32), ("kahan_degree", c_i32),
-                ("max_dim", c_i32), ("short_degree", c_i32), ("col_panels", c_i32)]
+                ("max_dim", c_i32), ("short_degree", c_i32), ("col_panels", c_i32),
+                ("row_order", c_i32)]
 
 
 class PlanTuning(ctypes.Structure):

@@ -482,7 +482,15 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
     all.push_back(w);
     if (mid < e) rem.push_back(w);
   };
-  for (int64_t i = 0; i < M; ++i) {
+  // Item order: row order, or rows by decreasing entry count (rows that
+  // share hub columns run together).
+  std::vector<int32_t> order(M);
+  std::iota(order.begin(), order.end(), 0);
+  if (o.row_order == 1)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t x, int32_t y) { return cnt[x + 1] - cnt[x] > cnt[y + 1] - cnt[y]; });
+  for (int64_t oi = 0; oi < M; ++oi) {
+    const int64_t i = order[oi];
     const int64_t b = cnt[i], e = cnt[i + 1], d = e - b, mid = b + lcnt[i];
     if (d == 0) {
       out->empty_rows.push_back((int32_t)i);

@@ -772,7 +772,8 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
   *out = nullptr;
   cola_plan_opts o{};
   if (opts) o = *opts;
-  if (o.hub_threshold < 0 || o.chunk_nnz < 0 || o.kahan_degree < 0 || o.max_dim < 0 || o.short_degree < 0)
+  if (o.hub_threshold < 0 || o.chunk_nnz < 0 || o.kahan_degree < 0 || o.max_dim < 0 || o.short_degree < 0 ||
+      o.col_panels < 0 || o.col_panels > 64 || o.row_order < 0 || o.row_order > 1)
     return fail(COLA_ERR_INVALID, "plan_create: negative option");
   cola::HostPlan hp;
   cola_status st = cola::build_plan(part->p, o, &hp);

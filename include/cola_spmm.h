@@ -244,6 +244,8 @@ typedef struct {
   int32_t col_panels;    /* K > 1: also build a column-panel schedule (local
                             columns and packed rows each cut into K ranges,
                             one pass per range) for cache reuse; 0 = none   */
+  int32_t row_order;     /* order of work items: 0 = row order, 1 = rows by
+                            decreasing entry count                          */
 } cola_plan_opts;
 cola_status cola_plan_create(cola_partition p, const cola_plan_opts *opts,
                              cola_plan *out);
