@@ -11,6 +11,13 @@ degree-binned row kernel + the hub reduction, C overwritten.
 value = algorithmic bytes of the partition SpMM / step time, with
   bytes_alg = 4(M+1) + nnz_p*(4+4) + (U_l + U_r)*N*s_B + M*N*4
 (each distinct B row read once, C written once).
+
+Extras (same partition unless stated): reduced-precision storage of B
+(BF16 / FP16 / FP8-e4m3 + row scales) at dims 64 / 128 / 256 with relative
+Frobenius and max-row error against the FP64 oracle (computed on the GPU in
+FP64 over sampled rows); the reference-semantics path (nnz_split + CSC
+scatter atomics); the batched form (64 SpMMs, one persistent launch); the
+served form (Poisson arrivals over 200 resident graphs).
 """
 import argparse
 import json
@@ -31,6 +38,88 @@ def bytes_alg(M, nnz, U, N, s_B, s_sc=0):
     return 4 * (M + 1) + nnz * 8 + U * (N * s_B + s_sc) + M * N * 4
 
 
+def timed(torch, fn, reps, warm=1):
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s0.record()
+    for _ in range(reps):
+        fn()
+    s1.record()
+    torch.cuda.synchronize()
+    return s0.elapsed_time(s1) / reps
+
+
+class Sample:
+    """FP64 oracle rows of the partition product for a fixed row sample, and
+    on-device FP64 error norms of C over those rows."""
+
+    def __init__(self, orc, torch, rp, ci, va, lb, M, Bh, n, count=512):
+        self.rows = np.unique(np.concatenate([np.arange(0, M, max(1, M // count)), [M - 1]]))
+        ref = np.stack([orc.spmm_f64(rp, ci, va, lb + int(i), lb + int(i) + 1, Bh[:n])[0] for i in self.rows])
+        self.ref = torch.from_numpy(ref).cuda()
+        self.idx = torch.from_numpy(self.rows.astype(np.int64)).cuda()
+
+    def errors(self, cola, C):
+        sub = C.index_select(0, self.idx).contiguous()
+        out = cola.error_norms(sub, self.ref).cpu().numpy()
+        return {"rel_fro": float(np.sqrt(out[0] / out[1])), "max_row_rel": float(np.sqrt(out[2]))}
+
+
+def precision_section(cola, torch, orc, log, ctx, args, f32_tuning):
+    """BF16 / FP16 / FP8-e4m3 storage of B at dims 64, 128, 256 (FP32
+    accumulation, Kahan on rows above 1024 entries), each autotuned, against
+    FP32 storage at the same dim."""
+    plan, P, M, n, rp, ci, va, lb = ctx
+    out = {}
+    dev = torch.device("cuda:0")
+    for dim in args.precision_dims:
+        gen = torch.Generator(device=dev).manual_seed(4321 + dim)
+        Bf = torch.rand((P * args.nparts, dim), generator=gen, device=dev, dtype=torch.float32).mul_(2).sub_(1)
+        sample = Sample(orc, torch, rp, ci, va, lb, M, Bf.cpu().numpy(), n, count=256)
+        per = {}
+        for dt in (cola.F32, cola.BF16, cola.F16, cola.FP8_E4M3):
+            name = cola.DTYPE_NAMES[dt]
+            if dt == cola.F32:
+                Bq, sc = Bf, None
+            else:
+                Bq, sc = cola.quantize_rows(Bf, dt)
+            tdt = {cola.F32: torch.float32, cola.BF16: torch.uint16, cola.F16: torch.uint16,
+                   cola.FP8_E4M3: torch.uint8}[dt]
+            pq = torch.empty((max(plan.U_r, 1), dim), dtype=tdt, device=dev)
+            psc = torch.empty(max(plan.U_r, 1), dtype=torch.float32, device=dev) if dt == cola.FP8_E4M3 else None
+            Bl = Bq[args.rank * P:(args.rank + 1) * P]
+            sl = None if sc is None else sc[args.rank * P:(args.rank + 1) * P]
+            C = torch.empty((P, dim), dtype=torch.float32, device=dev)
+            flags = cola.OVERWRITE | (cola.KAHAN if dt != cola.F32 else 0)
+            plan.pack(Bq, pq, dtype=dt, scales_global=sc, packed_scales=psc)
+            tq, _ = plan.autotune(C, Bl, pq, dtype=dt, flags=flags, iters=2, scales_local=sl, packed_scales=psc)
+
+            def step():
+                plan.pack(Bq, pq, dtype=dt, scales_global=sc, packed_scales=psc)
+                plan.spmm(C, Bl, pq, dtype=dt, flags=flags, scales_local=sl, packed_scales=psc)
+
+            ms = timed(torch, step, max(3, args.steps // 2))
+            s_sc = 4 if dt == cola.FP8_E4M3 else 0
+            rec = {"ms_per_step": ms,
+                   "GBps_alg": bytes_alg(M, plan.nnz, plan.U_l + plan.U_r, dim, cola.DTYPE_BYTES[dt], s_sc) /
+                   (ms * 1e-3) / 1e9,
+                   "tuning": tq}
+            rec.update(sample.errors(cola, C))
+            per[name] = rec
+            del Bq, sc, pq, psc, C
+        for name, rec in per.items():
+            rec["speedup_vs_f32"] = per["f32"]["ms_per_step"] / rec["ms_per_step"]
+        out[str(dim)] = per
+        log("precision dim %d: " % dim + ", ".join(
+            f"{k} {v['ms_per_step']:.3f} ms ({v['speedup_vs_f32']:.2f}x) err {v['rel_fro']:.1e}/{v['max_row_rel']:.1e}"
+            for k, v in per.items()))
+        del Bf, sample
+    plan.tuning = f32_tuning
+    return out
+
+
 def batched_section(cola, torch, log):
     """64 independent SpMMs (node count log-uniform in [2^12, 2^18], degree
     uniform in [4, 64], dim from {32, 64, 128, 256}, B in BF16) in one
@@ -38,7 +127,7 @@ def batched_section(cola, torch, log):
     from cola_amd import workloads
     t0 = time.time()
     specs = workloads.batched_specs(64, seed=7)
-    probs = workloads.build_problems(specs, dtype=cola.BF16)
+    probs = workloads.build_problems(specs, dtype=cola.BF16, plan_opts=dict(row_order=1))
     balg = sum(workloads.bytes_alg(p, 2) for p in probs)
     nnz = sum(p["nnz"] for p in probs)
     log(f"batched: 64 problems, {nnz} entries, built in {time.time() - t0:.1f}s")
@@ -47,25 +136,14 @@ def batched_section(cola, torch, log):
         for p in probs:
             p["plan"].spmm(p["C"], p["B"], None, dim=p["dim"], dtype=cola.BF16, flags=cola.OVERWRITE)
 
-    def timed(fn, reps):
-        fn()
-        torch.cuda.synchronize()
-        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        s0.record()
-        for _ in range(reps):
-            fn()
-        s1.record()
-        torch.cuda.synchronize()
-        return s0.elapsed_time(s1) / reps
-
-    ms_ind = timed(individual, 5)
+    ms_ind = timed(torch, individual, 5)
     batch = cola.Batch(probs, dtype=cola.BF16, flags=cola.OVERWRITE)
-    ms_untuned = timed(batch.run, 5)
+    ms_untuned = timed(torch, batch.run, 5)
     batch.autotune(iters=2)
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         batch.capture(stream)
-        ms_graph = timed(lambda: batch.replay(stream), 10)
+        ms_graph = timed(torch, lambda: batch.replay(stream), 10)
     torch.cuda.synchronize()
     tunings = {}
     for k, p in enumerate(probs):
@@ -88,7 +166,7 @@ def serving_section(cola, torch, log, args, dim_tunings):
     from cola_amd import workloads
     t0 = time.time()
     specs = workloads.batched_specs(args.serve_graphs, seed=11)
-    res = workloads.build_problems(specs, dtype=cola.BF16, keep_f32=True)
+    res = workloads.build_problems(specs, dtype=cola.BF16, keep_f32=True, plan_opts=dict(row_order=1))
     for p in res:
         del p["C"]
     resident_bytes = sum(p["plan"].info["device_bytes"] + p["B"].numel() * 2 + p["Bf"].numel() * 4 for p in res)
@@ -97,7 +175,7 @@ def serving_section(cola, torch, log, args, dim_tunings):
     arr, rid = workloads.poisson_trace(args.serve_rate, args.serve_seconds, len(res), seed=5)
     budget = 32 << 30
     out = {"resident_graphs": len(res), "rate_req_per_s": args.serve_rate, "trace_seconds": args.serve_seconds,
-           "memory_budget_bytes": budget}
+           "memory_budget_bytes": budget, "resident_bytes": resident_bytes}
     sv = cola.Server(budget, dtype=cola.BF16, max_batch=64, max_inflight=2, kahan=True)
     ids = [sv.add_resident(p["plan"], p["B"]) for p in res]
     for d, t in dim_tunings.items():
@@ -118,6 +196,34 @@ def serving_section(cola, torch, log, args, dim_tunings):
     return out
 
 
+def cpu_section(orc, log, rp, ci, va, lb, M, Bh, N, U_total, seconds):
+    """OpenMP FP32 CSR SpMM (the oracle's CPU port) over whole row blocks of
+    the partition until about `seconds` of CPU time."""
+    rows_per_block = max(1, M // 16)
+    done_rows = done_nnz = 0
+    t_cpu = 0.0
+    threads = 1
+    Cc = np.empty((rows_per_block, N), np.float32)
+    blk = 0
+    while t_cpu < seconds and done_rows < M:
+        r0 = lb + blk * rows_per_block
+        r1 = min(lb + M, r0 + rows_per_block)
+        tt = time.perf_counter()
+        _, threads = orc.spmm_omp_f32(rp, ci, va, r0, r1, Bh, Cc[:r1 - r0])
+        t_cpu += time.perf_counter() - tt
+        done_rows += r1 - r0
+        done_nnz += int(rp[r1] - rp[r0])
+        blk += 1
+    frac_rows = done_rows / M
+    cb = 4 * (done_rows + 1) + done_nnz * 8 + int(U_total * frac_rows) * N * 4 + done_rows * N * 4
+    cpu = {"value": cb / t_cpu / 1e9, "unit": "GB/s", "cores": int(threads), "kind": "port",
+           "sample": f"OpenMP FP32 CSR SpMM (oracle), rows [{lb},{lb + done_rows}) of the partition "
+                     f"({done_nnz} entries, {frac_rows:.0%} of its rows), {t_cpu:.1f}s on {threads} threads; "
+                     f"algorithmic bytes scaled by the sampled row fraction"}
+    log(f"cpu baseline {cpu['value']:.1f} GB/s on {threads} threads")
+    return cpu
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,9 +234,13 @@ def main():
     ap.add_argument("--nparts", type=int, default=8)
     ap.add_argument("--rank", type=int, default=0)
     ap.add_argument("--dim", type=int, default=128)
+    ap.add_argument("--row-order", type=int, default=1, help="plan item order: 0 row order, 1 by degree")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip every extra section")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--precision-dims", type=int, nargs="*", default=[64, 128, 256])
+    ap.add_argument("--no-reference-path", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-SpMM persistent-launch section")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
     ap.add_argument("--serve-graphs", type=int, default=200)
@@ -148,7 +258,6 @@ def main():
               "this build (the multi-GPU exchange is design-only).", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     import cola_amd as cola
@@ -164,16 +273,17 @@ def main():
     part = g.partition(args.nparts, args.rank)
     info = part.info()
     P, M, N = info["nodesPerPE"], info["num_rows"], args.dim
-    plan = cola.Plan(part)
+    plan = cola.Plan(part, row_order=args.row_order)
     nnz_p = plan.nnz
     U_l, U_r = plan.U_l, plan.U_r
+    lb = args.rank * P
     log(f"graph n={n} nnz={nnz_total} partition {args.rank}/{args.nparts}: M={M} nnz_p={nnz_p} "
         f"U_l={U_l} U_r={U_r} items={plan.info['num_items']} hubs={plan.info['num_hub_rows']} "
         f"setup {time.time() - t0:.1f}s")
 
     gen = torch.Generator(device=dev).manual_seed(1234)
     B = torch.rand((P * args.nparts, N), generator=gen, device=dev, dtype=torch.float32).mul_(2).sub_(1)
-    B_local = B[args.rank * P:(args.rank + 1) * P]
+    B_local = B[lb:lb + P]
     packed = torch.empty((max(U_r, 1), N), dtype=torch.float32, device=dev)
     C = torch.empty((P, N), dtype=torch.float32, device=dev)
 
@@ -204,12 +314,9 @@ def main():
         with open(args.save_tuning, "w") as f:
             json.dump(tuning, f)
 
-    def step():
+    for _ in range(args.warmup):
         plan.pack(B, packed)
         plan.spmm(C, B_local, packed, flags=cola.OVERWRITE)
-
-    for _ in range(args.warmup):
-        step()
     torch.cuda.synchronize()
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
     start = torch.cuda.Event(enable_timing=True)
@@ -224,8 +331,7 @@ def main():
         ev[k][2].record()
     end.record()
     torch.cuda.synchronize()
-    total_ms = start.elapsed_time(end)
-    ms = total_ms / args.steps
+    ms = start.elapsed_time(end) / args.steps
     pack_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
     spmm_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
     balg = bytes_alg(M, nnz_p, U_l + U_r, N, 4)
@@ -235,7 +341,7 @@ def main():
     kbytes = nnz_p * 8 + plan.info["num_items"] * 16 + (U_l + U_r) * N * 4 + M * N * 4
     k_gbs = kbytes / (spmm_ms * 1e-3) / 1e9
     log(f"step {ms:.3f} ms (pack {pack_ms:.3f} + spmm {spmm_ms:.3f}); {value:.0f} GB/s alg; "
-        f"spmm kernel {k_gbs:.0f} GB/s; gather {nnz_p * N * 4 / (spmm_ms * 1e-3) / 1e9:.0f} GB/s")
+        f"spmm kernel {k_gbs:.0f} GB/s; gathered rows {nnz_p * N * 4 / (spmm_ms * 1e-3) / 1e9:.0f} GB/s")
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "plan_rows_f32_rmat22_traffic.json")
@@ -243,122 +349,48 @@ def main():
         with open(tfile) as f:
             tj = json.load(f)
         # Counter-measured traffic holds only for the schedule it was measured with.
-        if tj.get("tuning") == tuning and N == 128 and args.log_n == 22 and args.rank == 0:
+        if (tj.get("tuning") == tuning and tj.get("row_order", 0) == args.row_order and N == 128
+                and args.log_n == 22 and args.rank == 0):
             traffic = tj.get("hbm_bytes_per_launch")
 
     extras = {}
-    if not args.no_extras:
-        # Correctness of this very run on a sample of rows (FP64 oracle).
+    if not (args.no_extras and args.no_cpu):
         from oracle import oracle as orc
         rp, ci, va = g.arrays()
+    if not args.no_extras:
         Bh = B.cpu().numpy()
-        Ch = C.cpu().numpy()
-        sample = np.arange(0, M, max(1, M // 512))
-        ref = np.stack([orc.spmm_f64(rp, ci, va, args.rank * P + int(i), args.rank * P + int(i) + 1, Bh[:n])[0]
-                        for i in sample])
-        err = float(np.linalg.norm(Ch[sample] - ref) / np.linalg.norm(ref))
-        extras["fp32_rel_fro_sample"] = err
-        log(f"fp32 sample rel_fro {err:.2e}")
-        # Reduced-precision storage of B (FP32 accumulation, Kahan on hub rows),
-        # each with its own autotuned schedule.
-        f32_tuning = plan.tuning
-        for dt, tdt in ((cola.BF16, torch.uint16), (cola.FP8_E4M3, torch.uint8)):
-            name = cola.DTYPE_NAMES[dt]
-            Bq, sc = cola.quantize_rows(B, dt)
-            pq = torch.empty((max(U_r, 1), N), dtype=tdt, device=dev)
-            psc = torch.empty(max(U_r, 1), dtype=torch.float32, device=dev) if dt == cola.FP8_E4M3 else None
-            Bq_l = Bq[args.rank * P:(args.rank + 1) * P]
-            sc_l = None if sc is None else sc[args.rank * P:(args.rank + 1) * P]
-            Cq = torch.empty_like(C)
-            plan.pack(Bq, pq, dtype=dt, scales_global=sc, packed_scales=psc)
-            tq, _ = plan.autotune(Cq, Bq_l, pq, dtype=dt, flags=cola.OVERWRITE | cola.KAHAN, iters=3,
-                                  scales_local=sc_l, packed_scales=psc)
-
-            def step_q():
-                plan.pack(Bq, pq, dtype=dt, scales_global=sc, packed_scales=psc)
-                plan.spmm(Cq, Bq_l, pq, dtype=dt, flags=cola.OVERWRITE | cola.KAHAN, scales_local=sc_l,
-                          packed_scales=psc)
-
-            for _ in range(args.warmup):
-                step_q()
-            torch.cuda.synchronize()
-            s2, e2 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            s2.record()
-            for _ in range(args.steps):
-                step_q()
-            e2.record()
-            torch.cuda.synchronize()
-            ms_q = s2.elapsed_time(e2) / args.steps
-            Cqh = Cq.cpu().numpy()
-            err_q = float(np.linalg.norm(Cqh[sample] - ref) / np.linalg.norm(ref))
-            s_sc = 4 if dt == cola.FP8_E4M3 else 0
-            extras[name] = {"ms_per_step": ms_q,
-                            "GBps_alg": bytes_alg(M, nnz_p, U_l + U_r, N, cola.DTYPE_BYTES[dt], s_sc) /
-                            (ms_q * 1e-3) / 1e9,
-                            "speedup_vs_fp32": ms / ms_q, "rel_fro_sample_vs_fp64": err_q, "tuning": tq}
-            log(f"{name} step {ms_q:.3f} ms ({ms / ms_q:.2f}x fp32), rel_fro {err_q:.2e}, tuning {tq}")
-            del Bq, pq, Cq, sc, psc
-        plan.tuning = f32_tuning
-        # Reference-semantics path: nnz_split virtual rows + CSC scatter atomics.
-        a = part.arrays()
-        tl, pl, _ = cola.nnz_split(a["csr_rowIdx_l"], a["csr_rowPtr_l"], len(a["csr_colIdx_l"]))
-        tr, pr, _ = cola.nnz_split(a["csc_colIdx_r"], a["csc_colPtr_r"], len(a["csc_rowIdx_r"]))
-        zl = cola.d_csr_zip(pl, a["csr_colIdx_l"], tl, a["csr_val_l"])
-        zr = cola.d_csc_zip(pr, a["csc_rowIdx_r"], tr, a["csc_val_r"])
-        Cr = torch.zeros_like(C)
-        for _ in range(2):
-            cola.dist_spmm_ppl_batch(Cr, B, packed, zl, zr, N, P, args.rank)
-        torch.cuda.synchronize()
-        s3, e3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        reps = max(3, args.steps // 4)
-        s3.record()
-        for _ in range(reps):
-            cola.dist_spmm_ppl_batch(Cr, B, packed, zl, zr, N, P, args.rank)
-        e3.record()
-        torch.cuda.synchronize()
-        ms_ref = s3.elapsed_time(e3) / reps
-        extras["reference_semantics_path"] = {"ms_per_step": ms_ref,
-                                              "GBps_alg": balg / (ms_ref * 1e-3) / 1e9}
-        log(f"reference-semantics path (atomics) {ms_ref:.3f} ms")
-        del Cr, zl, zr
-
-    dim_tunings = {}
-    if not args.no_batched:
-        extras["batched"], dim_tunings = batched_section(cola, torch, log)
-    if not args.no_serve:
-        extras["serving"] = serving_section(cola, torch, log, args, dim_tunings)
+        sample = Sample(orc, torch, rp, ci, va, lb, M, Bh, n)
+        extras["fp32_errors_sampled_rows"] = sample.errors(cola, C)
+        log(f"fp32 errors {extras['fp32_errors_sampled_rows']}")
+        del Bh, sample
+        if not args.no_precision:
+            extras["precision"] = precision_section(cola, torch, orc, log, (plan, P, M, n, rp, ci, va, lb), args,
+                                                    tuning)
+            per = extras["precision"].get(str(N), {})
+            if "bf16" in per:
+                extras["bf16_vs_f32_at_dim%d" % N] = {"speedup": per["bf16"]["speedup_vs_f32"],
+                                                       "rel_fro": per["bf16"]["rel_fro"]}
+        if not args.no_reference_path:
+            a = part.arrays()
+            tl, pl, _ = cola.nnz_split(a["csr_rowIdx_l"], a["csr_rowPtr_l"], len(a["csr_colIdx_l"]))
+            tr, pr, _ = cola.nnz_split(a["csc_colIdx_r"], a["csc_colPtr_r"], len(a["csc_rowIdx_r"]))
+            zl = cola.d_csr_zip(pl, a["csr_colIdx_l"], tl, a["csr_val_l"])
+            zr = cola.d_csc_zip(pr, a["csc_rowIdx_r"], tr, a["csc_val_r"])
+            Cr = torch.zeros_like(C)
+            ms_ref = timed(torch, lambda: cola.dist_spmm_ppl_batch(Cr, B, packed, zl, zr, N, P, args.rank),
+                           max(3, args.steps // 4), warm=2)
+            extras["reference_semantics_path"] = {"ms_per_step": ms_ref, "GBps_alg": balg / (ms_ref * 1e-3) / 1e9}
+            log(f"reference-semantics path (nnz_split + scatter atomics) {ms_ref:.3f} ms")
+            del Cr, zl, zr
+        dim_tunings = {}
+        if not args.no_batched:
+            extras["batched"], dim_tunings = batched_section(cola, torch, log)
+        if not args.no_serve:
+            extras["serving"] = serving_section(cola, torch, log, args, dim_tunings)
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as orc
-        rp, ci, va = g.arrays()
-        Bh = B.cpu().numpy()[:n]
-        lb = args.rank * P
-        # Bounded sample: whole row blocks of the partition until ~cpu-seconds.
-        rows_per_block = max(1, M // 16)
-        done_rows = 0
-        done_nnz = 0
-        t_cpu = 0.0
-        threads = 1
-        Cc = np.empty((rows_per_block, N), np.float32)
-        blk = 0
-        while t_cpu < args.cpu_seconds and done_rows < M:
-            r0 = lb + blk * rows_per_block
-            r1 = min(lb + M, r0 + rows_per_block)
-            tt = time.perf_counter()
-            _, threads = orc.spmm_omp_f32(rp, ci, va, r0, r1, Bh, Cc[:r1 - r0])
-            t_cpu += time.perf_counter() - tt
-            done_rows += r1 - r0
-            done_nnz += int(rp[r1] - rp[r0])
-            blk += 1
-        # Scale the algorithmic bytes to the sampled rows.
-        frac_rows = done_rows / M
-        cb = 4 * (done_rows + 1) + done_nnz * 8 + int((U_l + U_r) * frac_rows) * N * 4 + done_rows * N * 4
-        cpu = {"value": cb / t_cpu / 1e9, "unit": "GB/s", "cores": int(threads), "kind": "port",
-               "sample": f"OpenMP FP32 CSR SpMM (oracle), rows [{lb},{lb + done_rows}) of partition "
-                         f"{args.rank} ({done_nnz} nnz, {done_rows / M:.0%} of its rows), {t_cpu:.1f}s; "
-                         f"bytes scaled by the sampled row fraction"}
-        log(f"cpu baseline {cpu['value']:.1f} GB/s on {threads} threads")
+        cpu = cpu_section(orc, log, rp, ci, va, lb, M, B.cpu().numpy()[:n], N, U_l + U_r, args.cpu_seconds)
 
     out = {
         "metric": METRIC,
@@ -378,11 +410,11 @@ def main():
                    "nodes": n, "nnz_graph": nnz_total, "rows": M, "nnz_partition": nnz_p,
                    "unique_local_cols": U_l, "unique_remote_cols": U_r, "dim": N,
                    "bytes_alg": balg, "step": "pack + plan row kernel + hub reduce, C overwritten",
-                   "tuning": tuning},
+                   "row_order": args.row_order, "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": round(k_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "plan_rows_kernel (+ hub_reduce_kernel)", "kernel_ms": round(spmm_ms, 4),
-                     "kernel_bytes_alg": kbytes},
+                     "kernel": "plan_rows_kernel (+ hub_reduce_kernel), one cola_spmm_partition call",
+                     "kernel_ms": round(spmm_ms, 4), "kernel_bytes_alg": kbytes},
         "cpu_baseline": cpu,
         "kernels_ms": {"pack": round(pack_ms, 4), "spmm": round(spmm_ms, 4)},
         "extras": extras,
