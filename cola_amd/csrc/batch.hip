@@ -77,7 +77,7 @@ __global__ __launch_bounds__(256) void batch_rows_kernel(const BatchSeg *__restr
   int si = 0;
   for (int t = t0 + wave; t < t1; t += nwave) {
     while (si < n_segs - 1 && t >= segs[si].task_end) ++si;
-    const BatchSeg &sg = segs[si];
+    const BatchSeg sg = segs[si];  // copied once per task (wave-uniform)
     const int lt = t - sg.task_begin;
     const int tile = lt % sg.n_tiles, grp = lt / sg.n_tiles;
     const int glog = sg.g_log2, G = 1 << glog;
@@ -231,11 +231,14 @@ __global__ __launch_bounds__(256) void batch_hub_reduce_kernel(const HubSeg *__r
 template <int DT, int EPL>
 void launch_batch(const BatchSeg *segs, int n_segs, int total, int accumulate, bool kahan, int blocks,
                   hipStream_t s) {
+  // Wider elements per lane hold more accumulators: fewer loads in flight
+  // keep the register budget at >= 5 waves per SIMD.
+  constexpr int U = EPL >= 8 ? 4 : 8;
   if (kahan)
-    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, 8, true>), dim3(blocks), dim3(256), 0, s, segs, n_segs,
+    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, U, true>), dim3(blocks), dim3(256), 0, s, segs, n_segs,
                        total, accumulate);
   else
-    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, 8, false>), dim3(blocks), dim3(256), 0, s, segs, n_segs,
+    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, U, false>), dim3(blocks), dim3(256), 0, s, segs, n_segs,
                        total, accumulate);
 }
 

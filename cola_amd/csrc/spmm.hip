@@ -937,7 +937,32 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
   float bms = 0.f;
   st = time_of(best, &bms);
   const int ev = vec_elems(dtype);
-  // Coordinate descent over one knob at a time.
+  // Coupled knobs first: a small grid over column tile x grid size x
+  // long-row rows-per-wave x XCD remap.
+  for (int tile : {0, dim / 2, dim / 4}) {
+    if (st) break;
+    if (tile != 0 && (tile < ev || tile % ev != 0)) continue;
+    for (int bpc : {4, 8, 16})
+      for (int rl : {1, 2})
+        for (int x : {0, 1}) {
+          if (st) break;
+          cola_plan_tuning t = best;
+          t.col_tile = tile;
+          t.blocks_per_cu = bpc;
+          t.rpw_long = rl;
+          t.xcd_remap = x;
+          float ms = 0.f;
+          if ((st = time_of(t, &ms))) break;
+          if (verbose)
+            fprintf(stderr, "[autotune grid] tile=%d blocks_per_cu=%d rpw_long=%d xcd=%d: %.4f ms\n", tile, bpc, rl,
+                    x, ms);
+          if (ms < bms * 0.99f) {
+            bms = ms;
+            best = t;
+          }
+        }
+  }
+  // Then coordinate descent over one knob at a time.
   for (int pass = 0; pass < 2 && !st; ++pass) {
     std::vector<cola_plan_tuning> cands;
     for (int tile : {0, dim / 2, dim / 4, dim / 8}) {
