@@ -91,6 +91,9 @@ SIGNATURES = {
     "cola_graph_info": [c_vp, P_i64, P_i64, P_i64],
     "cola_graph_arrays": [c_vp, ctypes.POINTER(P_i64), ctypes.POINTER(P_i32), ctypes.POINTER(P_f32)],
     "cola_graph_destroy": [c_vp],
+    "cola_graph_from_mtx": [c_char_p, c_int, ctypes.POINTER(c_vp)],
+    "cola_graph_write_mtx": [c_vp, c_char_p, c_int],
+    "cola_graph_reorder": [c_vp, c_int, c_vp, ctypes.POINTER(c_vp)],
     "cola_row_partition": [c_vp, c_int, c_int, ctypes.POINTER(c_vp)],
     "cola_partition_from_arrays": [c_i64, c_int, c_int, ctypes.POINTER(PartitionArrays), c_i64, c_i64,
                                    c_i64, c_i64, ctypes.POINTER(c_vp)],

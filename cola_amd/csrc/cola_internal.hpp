@@ -29,6 +29,15 @@ struct Partition {
   std::vector<float> csc_val_r;
 };
 
+// Graph helpers shared by the generator and the MatrixMarket reader.
+// assign_values: 1/sqrt(d_i d_j) when gcn, else 1.0 for every entry.
+void assign_values(Graph *g, bool gcn);
+// Cuthill-McKee order, new -> old.
+std::vector<int32_t> cuthill_mckee(const Graph &g);
+// out = in relabelled (new vertex i = old new2old[i]); values carried when
+// in.val is non-empty; columns sorted within each row.
+void relabel(const Graph &in, const std::vector<int32_t> &new2old, Graph *out);
+
 // Validates the structural invariants of a partition (what the reference's
 // writer asserts, prep_torch_par.py:50,53,71,74, plus index ranges).
 cola_status validate_partition(const Partition &p);

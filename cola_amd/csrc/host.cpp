@@ -114,7 +114,7 @@ static void finish_rows(int64_t n, std::vector<int64_t> &start,
               g->col_idx.begin() + g->row_ptr[i]);
 }
 
-static void assign_values(Graph *g, bool gcn) {
+void assign_values(Graph *g, bool gcn) {
   const int64_t n = g->n;
   g->val.resize(g->col_idx.size());
   if (!gcn) {
@@ -135,7 +135,7 @@ static void assign_values(Graph *g, bool gcn) {
 // Cuthill-McKee order: components in order of their minimum-degree vertex,
 // breadth-first, neighbours visited by increasing (degree, id); isolated
 // vertices last.  Returns new->old.
-static std::vector<int32_t> cuthill_mckee(const Graph &g) {
+std::vector<int32_t> cuthill_mckee(const Graph &g) {
   const int64_t n = g.n;
   std::vector<int32_t> deg(n);
   for (int64_t i = 0; i < n; ++i) deg[i] = (int32_t)(g.row_ptr[i + 1] - g.row_ptr[i]);
@@ -173,9 +173,9 @@ static std::vector<int32_t> cuthill_mckee(const Graph &g) {
   return order;
 }
 
-static void relabel(const Graph &in, const std::vector<int32_t> &new2old,
-                    Graph *out) {
+void relabel(const Graph &in, const std::vector<int32_t> &new2old, Graph *out) {
   const int64_t n = in.n;
+  const bool has_val = !in.val.empty();
   std::vector<int32_t> old2new(n);
   for (int64_t i = 0; i < n; ++i) old2new[new2old[i]] = (int32_t)i;
   out->n = n;
@@ -185,14 +185,30 @@ static void relabel(const Graph &in, const std::vector<int32_t> &new2old,
     out->row_ptr[i + 1] = out->row_ptr[i] + (in.row_ptr[o + 1] - in.row_ptr[o]);
   }
   out->col_idx.resize(in.col_idx.size());
-#pragma omp parallel for schedule(dynamic, 1024)
-  for (int64_t i = 0; i < n; ++i) {
-    const int32_t o = new2old[i];
-    int64_t w = out->row_ptr[i];
-    for (int64_t k = in.row_ptr[o]; k < in.row_ptr[o + 1]; ++k)
-      out->col_idx[w++] = old2new[in.col_idx[k]];
-    std::sort(out->col_idx.begin() + out->row_ptr[i],
-              out->col_idx.begin() + out->row_ptr[i + 1]);
+  out->val.resize(has_val ? in.val.size() : 0);
+#pragma omp parallel
+  {
+    std::vector<std::pair<int32_t, float>> row;
+#pragma omp for schedule(dynamic, 1024)
+    for (int64_t i = 0; i < n; ++i) {
+      const int32_t o = new2old[i];
+      const int64_t w = out->row_ptr[i];
+      if (!has_val) {
+        for (int64_t k = in.row_ptr[o]; k < in.row_ptr[o + 1]; ++k)
+          out->col_idx[w + k - in.row_ptr[o]] = old2new[in.col_idx[k]];
+        std::sort(out->col_idx.begin() + w, out->col_idx.begin() + out->row_ptr[i + 1]);
+        continue;
+      }
+      row.clear();
+      for (int64_t k = in.row_ptr[o]; k < in.row_ptr[o + 1]; ++k)
+        row.emplace_back(old2new[in.col_idx[k]], in.val[k]);
+      std::sort(row.begin(), row.end(),
+                [](const auto &x, const auto &y) { return x.first < y.first; });
+      for (size_t k = 0; k < row.size(); ++k) {
+        out->col_idx[w + k] = row[k].first;
+        out->val[w + k] = row[k].second;
+      }
+    }
   }
 }
 

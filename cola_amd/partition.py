@@ -2,7 +2,9 @@
 and load path, backed by the C++ library).
 
 Reference behaviour mirrored (paths relative to the CoLa tree):
+  prep_torch_par.py:14-20   load_mtx       -> Graph.from_mtx
   prep_torch_par.py:24-79   row_partition  -> Partition.row_partition / Graph.partition
+  rabbit_order_write.py:46-77  reorder + mmwrite -> Graph.reorder / Graph.write_mtx
   prep_torch_par.py:93-110  serialize      -> serialize
   src/distSpMM_ppl_batch.cu:49-99  partition load + 1.0 values -> Partition.load
   include/dist_spmm_ppl_batch.cuh:7-27   bin2vec   -> bin2vec
@@ -72,6 +74,30 @@ class Graph:
         np.cumsum(counts, out=rp[1:])
         v = None if vals is None else np.asarray(vals, dtype=np.float32)[order]
         return cls.from_csr(rp, cols[order], v)
+
+    @classmethod
+    def from_mtx(cls, path, values="file"):
+        """Graph from a MatrixMarket coordinate file (what scipy.io.mmread +
+        tocsr() give: symmetric entries mirrored, duplicates summed).
+        values: "file" (the file's values; pattern files 1.0), "one" (every
+        entry 1.0, the reference driver's override) or "gcn"."""
+        mode = {"file": 0, "one": 1, "gcn": 2}[values]
+        h = ctypes.c_void_p()
+        check(lib().cola_graph_from_mtx(os.fsencode(path), mode, ctypes.byref(h)))
+        return cls(h)
+
+    def write_mtx(self, path, with_values=True):
+        """Writes a "coordinate real general" (or "pattern general") file."""
+        check(lib().cola_graph_write_mtx(self._h, os.fsencode(path), 1 if with_values else 0))
+
+    def reorder(self, method="cm"):
+        """Relabelled copy and its new->old permutation.  method: "cm"
+        (Cuthill-McKee), "rcm" (reversed) or "degree" (degree descending)."""
+        m = {"cm": 1, "rcm": 2, "degree": 3}[method]
+        perm = np.empty(self.n, dtype=np.int32)
+        h = ctypes.c_void_p()
+        check(lib().cola_graph_reorder(self._h, m, _ptr(perm), ctypes.byref(h)))
+        return Graph(h), perm
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib._lib is not None:
@@ -196,6 +222,32 @@ def serialize(graph, bin_path, graph_name, nGPU_list, write_values=True):
     for nGPU in nGPU_list:
         check(lib().cola_serialize(graph.handle, os.fsencode(bin_path), graph_name.encode(), int(nGPU),
                                    1 if write_values else 0))
+
+
+def load_mtx(graph_name, mtx_path, values="one"):
+    """<mtx_path>/<graph_name>.mtx as a Graph (prep_torch_par.py:14-20).
+    values="one" keeps the reference's unweighted view."""
+    path = os.path.join(mtx_path, f"{graph_name}.mtx")
+    return Graph.from_mtx(path, values=values)
+
+
+def process_graph(graph_name, mtx_path, bin_path, nGPU_list, write_values=False):
+    """load_mtx + serialize for every GPU count (prep_torch_par.py:93-118);
+    the partitions are written in the reference's layout under
+    <bin_path>/<graph_name>/GPU<n>/.  Returns the Graph."""
+    g = load_mtx(graph_name, mtx_path)
+    serialize(g, bin_path, graph_name, nGPU_list, write_values=write_values)
+    return g
+
+
+def reorder_mtx(src_path, dst_path, method="cm", with_values=False):
+    """Reads a MatrixMarket graph, relabels it for locality and writes the
+    relabelled graph (the flow of rabbit_order_write.py:60-77 with this
+    library's orders).  Returns the new->old permutation."""
+    g = Graph.from_mtx(src_path, values="file")
+    h, perm = g.reorder(method)
+    h.write_mtx(dst_path, with_values=with_values)
+    return perm
 
 
 def bin2vec(path, dtype=np.int32):

@@ -93,6 +93,48 @@ cola_status cola_graph_arrays(cola_graph g, const int64_t **row_ptr,
                               const int32_t **col_idx, const float **val);
 cola_status cola_graph_destroy(cola_graph g);
 
+/* Value policy when a graph is read from a MatrixMarket file. */
+enum {
+  COLA_MTX_VALUES_FILE = 0, /* the file's values (pattern files: 1.0)        */
+  COLA_MTX_VALUES_ONE = 1,  /* every stored entry 1.0 (drv:83,99 override)    */
+  COLA_MTX_VALUES_GCN = 2   /* 1/sqrt(d_i·d_j) over the deduplicated pattern */
+};
+
+/* Reads a MatrixMarket coordinate file into a global CSR graph: what
+ * load_mtx (prep_torch_par.py:14-20, scipy.io.mmread) followed by the
+ * writer's tocsr() conversions yields.  Fields real / double / integer /
+ * pattern; symmetry general / symmetric (off-diagonal entries mirrored) /
+ * skew-symmetric (mirrored negated).  Indices are 1-based in the file.
+ * Duplicates are summed in FP64 and rounded once to FP32.  The matrix must
+ * be square (rows = graph nodes).  Parsed in parallel on the host cores.
+ * COLA_ERR_IO: missing / unreadable / truncated file;
+ * COLA_ERR_UNSUPPORTED: array format, complex or hermitian fields, or a
+ * non-square matrix; COLA_ERR_INVALID: malformed header, size line or entry,
+ * index out of range, entry count different from the size line. */
+cola_status cola_graph_from_mtx(const char *path, int values, cola_graph *out);
+
+/* Writes the graph as "%%MatrixMarket matrix coordinate real general" (or
+ * "pattern general" when with_values == 0), 1-based, row-major order, values
+ * printed with 9 significant digits (FP32 round-trips exactly).  The writing
+ * half of rabbit_order_write.py:60-77 (scipy.io.mmwrite of the reordered
+ * graph). */
+cola_status cola_graph_write_mtx(cola_graph g, const char *path,
+                                 int with_values);
+
+/* Locality orders (the reference reorders with Rabbit Order,
+ * rabbit_order_write.py:46-55; this build's orders are its own). */
+enum {
+  COLA_ORDER_CUTHILL_MCKEE = 1, /* BFS from min-degree vertices, neighbours
+                                   by (degree, id); isolated vertices last */
+  COLA_ORDER_REVERSE_CM = 2,    /* the Cuthill-McKee order reversed         */
+  COLA_ORDER_DEGREE = 3         /* degree descending, ties by id           */
+};
+
+/* Relabels the graph (values carried along): new vertex i = old vertex
+ * new2old[i].  new2old (n entries) may be NULL. */
+cola_status cola_graph_reorder(cola_graph g, int method, int32_t *new2old,
+                               cola_graph *out);
+
 /* One rank's partition in the prep_torch_par.py:24-79 layout, host memory.
  * Array lengths: csr_rowPtr_l L+1, csr_colIdx_l/csr_val_l nnz_l,
  * csr_rowIdx_l L, csc_colPtr_r U_r+1, csc_rowIdx_r/csc_val_r nnz_r,

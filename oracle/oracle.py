@@ -12,6 +12,12 @@ Contents:
     scipy's coo -> csr/csc conversion.  Pinned by the structural invariants the
     reference asserts (prep_torch_par.py:50,53,71,74) and by brute-force
     reconstruction of the row panel (tests/test_oracle.py).
+  - par_files_from_mtx: prep_torch_par.py:14-101 for a MatrixMarket file,
+    restated step by step with the same third-party routines the reference
+    calls (scipy.io.mmread, coo_matrix(...).tocsr() / .tocsc(), np.unique):
+    the global files and every rank's six partition files as int32 arrays.
+    Pinned by the structural invariants of row_partition_np and by equality
+    with row_partition_np on the deduplicated CSR (tests/test_mtx.py).
   - dense_partition_product: brute-force dense A_p·B for tiny graphs.
 """
 import ctypes
@@ -190,6 +196,42 @@ def row_partition_np(row_ptr, col_idx, val, n, nGPU, rank):
                 csr_rowIdx_l=rowIdx_l.astype(np.int32), csr_val_l=lv.astype(np.float32),
                 csc_colPtr_r=colPtr_r.astype(np.int32), csc_rowIdx_r=rr.astype(np.int32),
                 csc_colIdx_r=colIdx_r.astype(np.int32), csc_val_r=rv.astype(np.float32))
+
+
+def par_files_from_mtx(path, nGPU):
+    """prep_torch_par.py:14-101 for one MatrixMarket file and one GPU count.
+
+    Returns {"global_rowPtr": .., "global_colIdx": .., (stem, rank): array}
+    with the int32 contents the reference writes (file name stems as in
+    prep_torch_par.py:56-58,77-79).  Values are not written by the reference.
+    """
+    import scipy.io as sio
+    from scipy.sparse import coo_matrix
+    A = sio.mmread(path)                                   # par.py:19
+    A = coo_matrix(A)
+    out = {}
+    g = A.tocsr()                                          # par.py:97
+    out["global_rowPtr"] = np.asarray(g.indptr).astype(np.int32)
+    out["global_colIdx"] = np.asarray(g.indices).astype(np.int32)
+    row, col = np.asarray(A.row), np.asarray(A.col)        # par.py:28-29
+    num_nodes = A.shape[0]
+    P = (num_nodes + nGPU - 1) // nGPU                     # par.py:31
+    for rank in range(nGPU):
+        lb, ub = rank * P, min((rank + 1) * P, num_nodes)
+        m = (row >= lb) & (row < ub)
+        r, c = row[m] - lb, col[m]
+        lm = (rank * P <= c) & (c < (rank + 1) * P)        # par.py:40
+        lr, lc = r[lm], c[lm]
+        L = coo_matrix((np.ones(len(lr)), (lr, lc % P)), shape=(ub - lb, P)).tocsr()
+        out[("csr_rowPtr_l", rank)] = np.unique(np.asarray(L.indptr, dtype=np.int32))
+        out[("csr_colIdx_l", rank)] = np.asarray(L.indices, dtype=np.int32)
+        out[("csr_rowIdx_l", rank)] = np.unique(lr).astype(np.int32)
+        rr, rc = r[~lm], c[~lm]
+        R = coo_matrix((np.ones(len(rr)), (rr, rc)), shape=(ub - lb, num_nodes)).tocsc()
+        out[("csc_colIdx_r", rank)] = np.unique(np.asarray(rc, dtype=np.int32))
+        out[("csc_colPtr_r", rank)] = np.unique(np.asarray(R.indptr, dtype=np.int32))
+        out[("csc_rowIdx_r", rank)] = np.asarray(R.indices, dtype=np.int32)
+    return out
 
 
 def dense_partition_product(row_ptr, col_idx, val, n, lb, ub, B):
