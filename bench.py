@@ -246,9 +246,10 @@ def main():
     ap.add_argument("--serve-graphs", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
-    ap.add_argument("--tuning", default="auto",
-                    help='"auto" (autotune before timing), a JSON object of plan tuning fields, or '
-                         '@file.json (its "tuning" object, e.g. a committed profile)')
+    ap.add_argument("--tuning", default="profile",
+                    help='"profile" (the schedule of the committed counter profile when it was taken on this '
+                         'workload, else autotune), "auto" (autotune before timing), a JSON object of plan '
+                         'tuning fields, or @file.json (its "tuning" object)')
     ap.add_argument("--save-tuning", default=None, help="write the FP32 tuning used to this JSON file")
     ap.add_argument("--copy-calibrate-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (counter calibration)")
@@ -297,11 +298,26 @@ def main():
         del src, dst
 
     plan.pack(B, packed)
-    if args.tuning == "auto":
+    # The committed counter profile (HBM bytes per launch) holds only for the
+    # workload and schedule it was measured with.
+    tfile = os.path.join(ROOT, "profiles", "plan_rows_f32_rmat22_traffic.json")
+    prof = None
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            prof = json.load(f)
+        if not (prof.get("row_order", 0) == args.row_order and N == 128 and args.log_n == 22
+                and args.degree == 32 and args.nparts == 8 and args.rank == 0):
+            prof = None
+    tuning_source = args.tuning
+    if args.tuning == "profile" and prof is None:
+        tuning_source = "auto"
+    if tuning_source == "auto":
         tuning, tune_ms = plan.autotune(C, B_local, packed, iters=3)
         log(f"autotuned f32: {tuning} ({tune_ms:.3f} ms)")
     else:
-        if args.tuning.startswith("@"):
+        if tuning_source == "profile":
+            tj = prof["tuning"]
+        elif args.tuning.startswith("@"):
             with open(os.path.join(ROOT, args.tuning[1:])) as f:
                 tj = json.load(f)
             tj = tj.get("tuning", tj)
@@ -309,7 +325,7 @@ def main():
             tj = json.loads(args.tuning)
         plan.tuning = tj
         tuning = plan.tuning
-        log(f"fixed f32 tuning: {tuning}")
+        log(f"fixed f32 tuning ({tuning_source}): {tuning}")
     if args.save_tuning:
         with open(args.save_tuning, "w") as f:
             json.dump(tuning, f)
@@ -343,15 +359,7 @@ def main():
     log(f"step {ms:.3f} ms (pack {pack_ms:.3f} + spmm {spmm_ms:.3f}); {value:.0f} GB/s alg; "
         f"spmm kernel {k_gbs:.0f} GB/s; gathered rows {nnz_p * N * 4 / (spmm_ms * 1e-3) / 1e9:.0f} GB/s")
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "plan_rows_f32_rmat22_traffic.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            tj = json.load(f)
-        # Counter-measured traffic holds only for the schedule it was measured with.
-        if (tj.get("tuning") == tuning and tj.get("row_order", 0) == args.row_order and N == 128
-                and args.log_n == 22 and args.rank == 0):
-            traffic = tj.get("hbm_bytes_per_launch")
+    traffic = prof.get("hbm_bytes_per_launch") if prof is not None and prof.get("tuning") == tuning else None
 
     extras = {}
     if not (args.no_extras and args.no_cpu):
@@ -410,9 +418,11 @@ def main():
                    "nodes": n, "nnz_graph": nnz_total, "rows": M, "nnz_partition": nnz_p,
                    "unique_local_cols": U_l, "unique_remote_cols": U_r, "dim": N,
                    "bytes_alg": balg, "step": "pack + plan row kernel + hub reduce, C overwritten",
-                   "row_order": args.row_order, "tuning": tuning},
+                   "row_order": args.row_order, "tuning": tuning, "tuning_source": tuning_source},
         "roofline": {"bound": "hbm", "achieved": round(k_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_GBps": round(traffic / (spmm_ms * 1e-3) / 1e9, 1) if traffic else None,
+                     "traffic_frac": round(traffic / (spmm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
                      "kernel": "plan_rows_kernel (+ hub_reduce_kernel), one cola_spmm_partition call",
                      "kernel_ms": round(spmm_ms, 4), "kernel_bytes_alg": kbytes},
         "cpu_baseline": cpu,

@@ -12,8 +12,8 @@
 // taking every (S/rpw)-th entry, U loads in flight per group, and the groups
 // of a team are summed with cross-lane shuffles at the end.  rpw = 1 is one
 // item per wave (long rows); rpw = S is one item per group (short rows).
-// Entry (column, value) pairs are fetched with one coalesced load per team
-// and per 64/rpw entries and broadcast with ds_bpermute.
+// Entry (column, value) pairs are staged K per team lane (K = U/G when U > G,
+// else 1) with coalesced loads and broadcast with ds_bpermute.
 #pragma once
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -190,33 +190,46 @@ __global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
 #pragma unroll
       for (int q = 0; q < EPL; ++q) acc[k][q] = err[k][q] = 0.f;
 
-    // Entries are taken in chunks of tsize: one coalesced load of (column,
-    // value) per team lane, the next chunk prefetched while this one is used.
-    int mc = 0, mc_next = 0;
-    float mv = 0.f, mv_next = 0.f;
-    if (b + tl < e) {
-      mc = a.cols[b + tl];
-      mv = a.vals[b + tl];
-    }
-    for (int base = b; base < e; base += tsize) {
-      const int n = min(tsize, e - base);
-      const int nb = base + tsize;
-      if (nb + tl < e) {
-        mc_next = a.cols[nb + tl];
-        mv_next = a.vals[nb + tl];
+    // Entries are taken in chunks of K*tsize: each team lane stages K
+    // (column, value) pairs with coalesced loads, the next chunk prefetched
+    // while this one is used.  A lane whose entry lies past the item's end
+    // stages the last entry's column with value 0, so every broadcast slot
+    // holds a valid row address and no load or FMA is predicated.
+    constexpr int K = U > G ? U / G : 1;  // pairs per lane: one batch of U per group fills K*tsize
+    const int csize = K * tsize;
+    int mc[K], mc_next[K];
+    float mv[K], mv_next[K];
+    auto stage = [&](int base, int (&c)[K], float (&v)[K]) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int idx = base + k * tsize + tl;
+        const int cl = min(idx, e - 1);
+        c[k] = a.cols[cl];
+        const float x = a.vals[cl];
+        v[k] = idx < e ? x : 0.f;
       }
+    };
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      mc[k] = mc_next[k] = 0;
+      mv[k] = mv_next[k] = 0.f;
+    }
+    if (b < e) stage(b, mc, mv);
+    for (int base = b; base < e; base += csize) {
+      const int n = min(csize, e - base);
+      if (base + csize < e) stage(base + csize, mc_next, mv_next);
       for (int j = 0; j < n; j += gpt * U) {
         // Broadcast the batch's (column, value) pairs first, then issue every
-        // load.  Slots past the end re-read the last entry's row (a valid,
-        // just-fetched address) with weight 0, so no load is predicated.
+        // load.  With K > 1 the batch is the whole chunk (j = 0) and slot u
+        // reads pair u / G of team lane (u % G) * gpt + tsub.
         int cu[U];
         float vu[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          const int ei = j + u * gpt + tsub;
-          const int src = (tbase + min(ei, n - 1)) << 2;
-          cu[u] = __builtin_amdgcn_ds_bpermute(src, mc);
-          vu[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv)));
+          const int k = K > 1 ? u / G : 0;
+          const int src = (tbase + (K > 1 ? (u % G) * gpt + tsub : j + u * gpt + tsub)) << 2;
+          cu[u] = __builtin_amdgcn_ds_bpermute(src, mc[k]);
+          vu[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv[k])));
         }
         Raw raw[U][NV];
         float av[U];
@@ -235,7 +248,7 @@ __global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
         for (int u = 0; u < U; ++u) {
           float v = vu[u];
           if constexpr (DT == COLA_FP8_E4M3) v *= (cu[u] >= 0 ? a.sl[cu[u]] : a.sp[~cu[u]]);
-          av[u] = (j + u * gpt + tsub < n) ? v : 0.f;
+          av[u] = v;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u)
@@ -258,8 +271,11 @@ __global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
             }
           }
       }
-      mc = mc_next;
-      mv = mv_next;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        mc[k] = mc_next[k];
+        mv[k] = mv_next[k];
+      }
     }
     // Sum the groups of each team (xor offsets below tsize stay in the team).
     for (int off = G; off < tsize; off <<= 1) {

@@ -933,34 +933,51 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
   };
   const char *env = getenv("COLA_AUTOTUNE_VERBOSE");
   const bool verbose = env && env[0] == '1';
+  // Every distinct configuration timed so far, for the final re-timing.
+  std::vector<std::pair<float, cola_plan_tuning>> seen;
+  auto same = [](const cola_plan_tuning &x, const cola_plan_tuning &y) {
+    return std::memcmp(&x, &y, sizeof(x)) == 0;
+  };
   cola_plan_tuning best = p->tune;
   float bms = 0.f;
-  st = time_of(best, &bms);
+  auto consider = [&](const cola_plan_tuning &t, const char *tag) -> cola_status {
+    for (const auto &e : seen)
+      if (same(e.second, t)) return COLA_OK;
+    float ms = 0.f;
+    cola_status r = time_of(t, &ms);
+    if (r) return r;
+    seen.emplace_back(ms, t);
+    if (verbose)
+      fprintf(stderr,
+              "[autotune %s] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d split=%d unroll=%d "
+              "panels=%d: %.4f ms\n",
+              tag, t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.xcd_remap, t.split_remote, t.unroll,
+              t.panels, ms);
+    if (seen.size() == 1 || ms < bms * 0.99f) {
+      bms = ms;
+      best = t;
+    }
+    return COLA_OK;
+  };
+  st = consider(best, "start");
   const int ev = vec_elems(dtype);
-  // Coupled knobs first: a small grid over column tile x grid size x
-  // long-row rows-per-wave x XCD remap.
+  // Coupled knobs first: a grid over column tile x grid size x long-row
+  // rows-per-wave x XCD remap x loads in flight.
   for (int tile : {0, dim / 2, dim / 4}) {
-    if (st) break;
     if (tile != 0 && (tile < ev || tile % ev != 0)) continue;
-    for (int bpc : {4, 8, 16})
-      for (int rl : {1, 2})
-        for (int x : {0, 1}) {
-          if (st) break;
-          cola_plan_tuning t = best;
-          t.col_tile = tile;
-          t.blocks_per_cu = bpc;
-          t.rpw_long = rl;
-          t.xcd_remap = x;
-          float ms = 0.f;
-          if ((st = time_of(t, &ms))) break;
-          if (verbose)
-            fprintf(stderr, "[autotune grid] tile=%d blocks_per_cu=%d rpw_long=%d xcd=%d: %.4f ms\n", tile, bpc, rl,
-                    x, ms);
-          if (ms < bms * 0.99f) {
-            bms = ms;
-            best = t;
+    for (int bpc : {1, 2, 4, 8})
+      for (int rl : {1, 2, 4, 8})
+        for (int x : {0, 1})
+          for (int u : {8, 16}) {
+            if (st) break;
+            cola_plan_tuning t = best;
+            t.col_tile = tile;
+            t.blocks_per_cu = bpc;
+            t.rpw_long = rl;
+            t.xcd_remap = x;
+            t.unroll = u;
+            st = consider(t, "grid");
           }
-        }
   }
   // Then coordinate descent over one knob at a time.
   for (int pass = 0; pass < 2 && !st; ++pass) {
@@ -976,12 +993,12 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
       t.rpw_short = r;
       cands.push_back(t);
     }
-    for (int r : {1, 2, 4}) {
+    for (int r : {1, 2, 4, 8}) {
       cola_plan_tuning t = best;
       t.rpw_long = r;
       cands.push_back(t);
     }
-    for (int b : {2, 4, 8, 16}) {
+    for (int b : {1, 2, 3, 4, 8, 16}) {
       cola_plan_tuning t = best;
       t.blocks_per_cu = b;
       cands.push_back(t);
@@ -1007,19 +1024,36 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
         t.panels = x;
         cands.push_back(t);
       }
-    for (const auto &t : cands) {
-      float ms = 0.f;
-      if ((st = time_of(t, &ms))) break;
-      if (verbose)
-        fprintf(stderr,
-                "[autotune] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d split=%d unroll=%d panels=%d: "
-                "%.4f ms\n",
-                t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.xcd_remap, t.split_remote, t.unroll, t.panels,
-                ms);
-      if (ms < bms * 0.99f) {
-        bms = ms;
-        best = t;
+    for (const auto &t : cands)
+      if ((st = consider(t, "descent"))) break;
+  }
+  // One timing per configuration is noisy: re-time the fastest few, three
+  // interleaved rounds each, and keep the best minimum.
+  if (!st) {
+    std::stable_sort(seen.begin(), seen.end(),
+                     [](const auto &x, const auto &y) { return x.first < y.first; });
+    const size_t k = std::min<size_t>(seen.size(), 5);
+    std::vector<float> score(k, 3.4e38f);
+    for (int round = 0; round < 3 && !st; ++round)
+      for (size_t i = 0; i < k && !st; ++i) {
+        float ms = 0.f;
+        if ((st = time_of(seen[i].second, &ms))) break;
+        score[i] = std::min(score[i], ms);
       }
+    if (!st) {
+      size_t w = 0;
+      for (size_t i = 1; i < k; ++i)
+        if (score[i] < score[w]) w = i;
+      best = seen[w].second;
+      bms = score[w];
+      if (verbose)
+        for (size_t i = 0; i < k; ++i)
+          fprintf(stderr,
+                  "[autotune final] #%zu tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d unroll=%d: "
+                  "%.4f ms%s\n",
+                  i, seen[i].second.col_tile, seen[i].second.rpw_short, seen[i].second.rpw_long,
+                  seen[i].second.blocks_per_cu, seen[i].second.xcd_remap, seen[i].second.unroll, score[i],
+                  i == w ? " <- best" : "");
     }
   }
   if (!st) st = run_plan(p, c, best, s);

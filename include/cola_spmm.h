@@ -320,11 +320,13 @@ typedef struct {
 cola_status cola_plan_set_tuning(cola_plan plan, const cola_plan_tuning *t);
 cola_status cola_plan_get_tuning(cola_plan plan, cola_plan_tuning *t);
 
-/* Project-written autotuner: times candidate launch parameters (column tile,
- * rows per wave of each bin, grid size, XCD remap) by coordinate descent on
- * the caller's buffers with HIP events, keeps the fastest in the plan and
- * returns it (and its time).  C is used as scratch: on return it holds
- * A_p·B (overwrite).  Blocks the calling thread. */
+/* Project-written autotuner: times candidate launch parameters on the
+ * caller's buffers with HIP events — a grid over the coupled knobs (column
+ * tile x grid size x long-bin rows per wave x XCD remap x loads in
+ * flight), then coordinate descent over every knob, then three interleaved
+ * re-timings of the five fastest distinct candidates — keeps the fastest in
+ * the plan and returns it (and its time).  C is used as scratch: on return it holds A_p·B
+ * (overwrite).  Blocks the calling thread. */
 cola_status cola_plan_autotune(cola_plan plan, float *C, int64_t ld_C,
                                const void *B_local, int64_t ld_B,
                                const float *scales_local, const void *packed,

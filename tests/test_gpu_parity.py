@@ -231,6 +231,36 @@ def test_hub_rows_split(hub_threshold, chunk):
     assert rel_fro(C2[:case.M], case.ref) <= FP32_TOL
 
 
+@pytest.mark.parametrize("tuning", [dict(col_tile=0, rpw_long=1, unroll=8), dict(col_tile=32, rpw_long=8, unroll=16),
+                                    dict(col_tile=16, rpw_long=4, unroll=16, split_remote=1)])
+def test_hub_rows_kahan_under_schedules(tuning):
+    """Hub chunks, Kahan sums and the multi-pair staging of 16 loads in
+    flight: partial rows and padding slots leave the result unchanged."""
+    n = 5000
+    rng = np.random.default_rng(3)
+    hub = np.zeros(n - 1, np.int64)
+    leaves = np.arange(1, n)
+    r = rng.integers(1, n, 20000)
+    c = rng.integers(1, n, 20000)
+    rows = np.concatenate([hub, leaves, r, c])
+    cols = np.concatenate([leaves, hub, c, r])
+    vals = rng.uniform(-1, 1, len(rows)).astype(np.float32)
+    g = cola.Graph.from_coo(n, rows, cols, vals)
+    case = Case(g, 2, 0, 64, seed=5)
+    plan = cola.Plan(case.part, hub_threshold=300, chunk_nnz=96, kahan_degree=8)
+    assert plan.info["num_hub_rows"] >= 1
+    full = dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=0, split_remote=0, unroll=8, panels=0)
+    full.update(tuning)
+    plan.tuning = full
+    packed = torch.empty((max(plan.U_r, 1), case.dim), dtype=torch.float32, device=case.dev)
+    plan.pack(case.B, packed)
+    for flags in (cola.OVERWRITE, cola.OVERWRITE | cola.KAHAN):
+        C = torch.full((case.P, case.dim), 2.0, dtype=torch.float32, device=case.dev)
+        plan.spmm(C, case.B_local(), packed, flags=flags)
+        torch.cuda.synchronize()
+        assert rel_fro(C.cpu().numpy()[:case.M], case.ref) <= FP32_TOL, flags
+
+
 def test_linearity_in_B(c1):
     """A·(B1 + B2) = A·B1 + A·B2 to FP32 rounding."""
     plan = cola.Plan(c1.part)
@@ -369,7 +399,12 @@ TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=
            dict(col_tile=32, rpw_short=8, rpw_long=2, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=16),
            dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=8, panels=1),
            dict(col_tile=32, rpw_short=0, rpw_long=2, blocks_per_cu=4, xcd_remap=0, split_remote=0, unroll=8,
-                panels=1)]
+                panels=1),
+           # 16 loads in flight on groups narrower than 16 lanes: each lane
+           # stages 2, 4 or 8 (column, value) pairs per chunk
+           dict(col_tile=32, rpw_short=8, rpw_long=8, blocks_per_cu=2, xcd_remap=0, split_remote=0, unroll=16),
+           dict(col_tile=16, rpw_short=16, rpw_long=8, blocks_per_cu=3, xcd_remap=1, split_remote=1, unroll=16),
+           dict(col_tile=8, rpw_short=32, rpw_long=4, blocks_per_cu=2, xcd_remap=0, split_remote=0, unroll=16)]
 
 
 @pytest.mark.parametrize("tuning", TUNINGS)
