@@ -945,9 +945,16 @@ cola_status cola_server_run_trace(cola_server sv, int64_t n, const double *arriv
       }
     }
     if (!progress) {
-      if (next < n && pending.empty()) {
+      // Sleep only while nothing is on the device: with a launch in flight
+      // the loop keeps polling its event, so a completion is seen as soon as
+      // it happens rather than at the next arrival.
+      bool inflight = false;
+      for (const auto &sl : sv->slots) inflight = inflight || sl.busy;
+      if (!inflight && next < n && pending.empty()) {
         const double wait = arrival_us[next] - (now_us() - t0);
         if (wait > 200.0) std::this_thread::sleep_for(std::chrono::microseconds((int64_t)(wait - 100.0)));
+      } else if (inflight) {
+        std::this_thread::yield();
       }
     }
   }
