@@ -5,11 +5,12 @@
 // Work list: the batch is a table of segments; a segment is one bin (long
 // rows + hub chunks, short rows, or empty rows for overwrite) of one
 // problem, cut into tasks of `rpw` consecutive items x `n_tiles` column
-// tiles.  Tasks are numbered consecutively across segments; each XCD walks a
-// contiguous eighth of the task numbers and its waves stride through it, so
-// one wave moves monotonically through the segment table.  Within a task the
-// lane layout is the plan path's (rows.hpp), with the row-group width G a
-// run-time power of two taken from the segment (dims differ per problem).
+// tiles (tile-major).  Tasks are numbered consecutively across segments;
+// each XCD walks a contiguous eighth of the task numbers and its waves stride
+// through it, so one wave moves monotonically through the segment table.  A
+// task runs the plan kernel's step (rows.hpp row_step) with the segment's
+// row-group width G compiled in, selected per task by a wave-uniform switch
+// (dims differ per problem).
 // Every item is reduced by one team in a fixed order, so a problem's result
 // does not depend on which other problems share the launch.
 #include <hip/hip_runtime.h>
@@ -48,6 +49,7 @@ struct BatchSeg {
   float *partials;
   int64_t pstride;
   int32_t n_items, dim, rpw, n_tiles, tile_w, g_log2, task_begin, task_end;
+  int32_t n_grp;  // row groups of rpw items: tasks = n_grp x n_tiles
 };
 
 struct HubSeg {
@@ -60,139 +62,63 @@ struct HubSeg {
   int64_t thread_begin;
 };
 
-template <int DT, int EPL, int U, bool KAHAN>
+// First task of each XCD's range (9 entries), chosen on the host so the
+// ranges carry equal estimated work (entries x columns + a per-item share).
+struct XcdBounds {
+  int32_t v[9];
+};
+
+template <int DT, int EPL, bool KAHAN>
 __global__ __launch_bounds__(256) void batch_rows_kernel(const BatchSeg *__restrict__ segs, int n_segs,
-                                                         int total, int accumulate) {
-  using V = Vec<DT, EPL>;
-  using T = typename V::T;
-  using Raw = typename V::Raw;
-  const int lane = threadIdx.x & 63;
+                                                         const XcdBounds xb, int accumulate) {
+  using T = typename Vec<DT, EPL>::T;
   const int wpb = blockDim.x >> 6;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int xcd = blockIdx.x & 7;
   const int blk = blockIdx.x >> 3;
   const int nblk = (gridDim.x - xcd + 7) >> 3;
-  const int t0 = (int)((int64_t)total * xcd / 8), t1 = (int)((int64_t)total * (xcd + 1) / 8);
+  const int t0 = xb.v[xcd], t1 = xb.v[xcd + 1];
   const int wave = blk * wpb + wid, nwave = nblk * wpb;
   int si = 0;
   for (int t = t0 + wave; t < t1; t += nwave) {
     while (si < n_segs - 1 && t >= segs[si].task_end) ++si;
-    const BatchSeg sg = segs[si];  // copied once per task (wave-uniform)
+    const BatchSeg &sg = segs[si];
+    // Tile-major within a segment: the waves sweep every row group of one
+    // column tile before the next, as the plan path's tile passes do.
     const int lt = t - sg.task_begin;
-    const int tile = lt % sg.n_tiles, grp = lt / sg.n_tiles;
-    const int glog = sg.g_log2, G = 1 << glog;
-    const int S = 64 >> glog;
-    const int rpw = sg.rpw, gpt = S / rpw, tsize = 64 / rpw;
-    const int sub = lane >> glog, gl = lane & (G - 1);
-    const int team = sub / gpt, tsub = sub % gpt;
-    const int tbase = team * tsize, tl = lane - tbase;
+    const int tile = lt / sg.n_grp, grp = lt - tile * sg.n_grp;
     const int c0 = tile * sg.tile_w;
-    const int w = min(sg.tile_w, sg.dim - c0);
-    const int col = gl * EPL;
-    const int colc = col < w ? col : 0;
-    const T *Bl = static_cast<const T *>(sg.B) + c0;
-    const T *Bp = sg.Bp ? static_cast<const T *>(sg.Bp) + c0 : nullptr;
-    const int it = grp * rpw + team;
-    const bool has = it < sg.n_items;
-    int4 item = make_int4(0, 0, 0, 0);
-    if (has) item = sg.items[it];
-    const int b = item.y, e = item.z;
-    const bool comp = KAHAN && (item.w & 1);
-    float acc[EPL], err[EPL];
-#pragma unroll
-    for (int q = 0; q < EPL; ++q) acc[q] = err[q] = 0.f;
-    int mc = 0, mc_next = 0;
-    float mv = 0.f, mv_next = 0.f;
-    if (b + tl < e) {
-      mc = sg.cols[b + tl];
-      mv = sg.vals[b + tl];
-    }
-    for (int base = b; base < e; base += tsize) {
-      const int n = min(tsize, e - base);
-      const int nb = base + tsize;
-      if (nb + tl < e) {
-        mc_next = sg.cols[nb + tl];
-        mv_next = sg.vals[nb + tl];
-      }
-      for (int j = 0; j < n; j += gpt * U) {
-        int cu[U];
-        float vu[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int src = (tbase + min(j + u * gpt + tsub, n - 1)) << 2;
-          cu[u] = __builtin_amdgcn_ds_bpermute(src, mc);
-          vu[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv)));
-        }
-        Raw raw[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int c = cu[u];
-          const bool loc = c >= 0;
-          const T *rp = (loc ? Bl : Bp) + (int64_t)(loc ? c : ~c) * (loc ? sg.ldB : sg.ldP);
-          raw[u] = V::load(rp + colc);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          float v = vu[u];
-          if constexpr (DT == COLA_FP8_E4M3) v *= (cu[u] >= 0 ? sg.sl[cu[u]] : sg.sp[~cu[u]]);
-          v = (j + u * gpt + tsub < n) ? v : 0.f;
-          float x[EPL];
-          V::widen(raw[u], x);
-          if (!comp) {
-#pragma unroll
-            for (int q = 0; q < EPL; ++q) acc[q] = fmaf(v, x[q], acc[q]);
-          } else {
-#pragma unroll
-            for (int q = 0; q < EPL; ++q) {
-              const float y = fmaf(v, x[q], -err[q]);
-              const float tt = acc[q] + y;
-              err[q] = (tt - acc[q]) - y;
-              acc[q] = tt;
-            }
-          }
-        }
-      }
-      mc = mc_next;
-      mv = mv_next;
-    }
-    for (int off = G; off < tsize; off <<= 1) {
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) {
-        const float s2 = __shfl_xor(acc[q], off);
-        const float e2 = __shfl_xor(err[q], off);
-        if (!comp) {
-          acc[q] += s2;
-        } else {
-          const float s1 = acc[q];
-          const float tt = s1 + s2;
-          const float lost = fabsf(s1) >= fabsf(s2) ? (s1 - tt) + s2 : (s2 - tt) + s1;
-          acc[q] = tt;
-          err[q] = err[q] + e2 - lost;
-        }
-      }
-    }
-    if (has && tsub == 0 && col < w) {
-      float *dst;
-      bool add_old;
-      if (item.x >= 0) {
-        dst = sg.C + (int64_t)item.x * sg.ldC + c0 + col;
-        add_old = accumulate != 0;
-      } else {
-        dst = sg.partials + (int64_t)(-item.x - 1) * sg.pstride + c0 + col;
-        add_old = false;
-      }
-#pragma unroll
-      for (int q = 0; q < EPL; q += 4) {
-        float4 o = make_float4(comp ? acc[q] - err[q] : acc[q], comp ? acc[q + 1] - err[q + 1] : acc[q + 1],
-                               comp ? acc[q + 2] - err[q + 2] : acc[q + 2],
-                               comp ? acc[q + 3] - err[q + 3] : acc[q + 3]);
-        float4 *d4 = reinterpret_cast<float4 *>(dst + q);
-        if (add_old) {
-          const float4 old = *d4;
-          o.x += old.x; o.y += old.y; o.z += old.z; o.w += old.w;
-        }
-        *d4 = o;
-      }
+    cola_dev::RowArgs a;
+    a.items = sg.items;
+    a.xcd_begin = nullptr;
+    a.cols = sg.cols;
+    a.vals = sg.vals;
+    a.Bl = sg.B ? static_cast<const T *>(sg.B) + c0 : nullptr;
+    a.ldB = sg.ldB;
+    a.sl = sg.sl;
+    a.Bp = sg.Bp ? static_cast<const T *>(sg.Bp) + c0 : nullptr;
+    a.ldP = sg.ldP;
+    a.sp = sg.sp;
+    a.C = sg.C + c0;
+    a.ldC = sg.ldC;
+    a.partials = sg.partials ? sg.partials + c0 : nullptr;
+    a.pstride = sg.pstride;
+    a.dim = min(sg.tile_w, sg.dim - c0);
+    a.accumulate = accumulate;
+    a.kahan = KAHAN ? 1 : 0;
+    a.rpw = sg.rpw;
+    a.xcd_remap = 0;
+    a.phase = 0;
+    const int it0 = grp * sg.rpw;
+    // The lane-group width is wave-uniform (one segment per task): each case
+    // is the plan kernel's step with that width compiled in.
+    switch (sg.g_log2) {
+      case 1: cola_dev::row_step<DT, EPL, 2, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
+      case 2: cola_dev::row_step<DT, EPL, 4, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
+      case 3: cola_dev::row_step<DT, EPL, 8, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
+      case 4: cola_dev::row_step<DT, EPL, 16, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
+      case 5: cola_dev::row_step<DT, EPL, 32, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
+      default: cola_dev::row_step<DT, EPL, 64, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
     }
   }
 }
@@ -229,17 +155,64 @@ __global__ __launch_bounds__(256) void batch_hub_reduce_kernel(const HubSeg *__r
 }
 
 template <int DT, int EPL>
-void launch_batch(const BatchSeg *segs, int n_segs, int total, int accumulate, bool kahan, int blocks,
+void launch_batch(const BatchSeg *segs, int n_segs, const XcdBounds &xb, int accumulate, bool kahan, int blocks,
                   hipStream_t s) {
-  // Wider elements per lane hold more accumulators: fewer loads in flight
-  // keep the register budget at >= 5 waves per SIMD.
-  constexpr int U = EPL >= 8 ? 4 : 8;
   if (kahan)
-    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, U, true>), dim3(blocks), dim3(256), 0, s, segs, n_segs,
-                       total, accumulate);
+    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, true>), dim3(blocks), dim3(256), 0, s, segs, n_segs, xb,
+                       accumulate);
   else
-    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, U, false>), dim3(blocks), dim3(256), 0, s, segs, n_segs,
-                       total, accumulate);
+    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, false>), dim3(blocks), dim3(256), 0, s, segs, n_segs, xb,
+                       accumulate);
+}
+
+// Cost model of a segment for the XCD split: a task (one row group of one
+// column tile) costs the entries of its items plus 8 per item, times the
+// tile's columns; prefix = the bin's per-item cost prefix (n_items + 1).
+struct SegCost {
+  const std::vector<int64_t> *prefix;
+  int64_t task_begin, task_end;
+  int32_t n_grp, rpw, tile_w, dim, n_items;
+};
+
+// Cumulative cost of the first k tasks of a segment.
+static double seg_cost_before(const SegCost &c, int64_t k) {
+  const int64_t tile = k / c.n_grp, g = k % c.n_grp;
+  const std::vector<int64_t> &P = *c.prefix;
+  const double full = (double)P[c.n_items];
+  const double cols_before = (double)std::min<int64_t>(tile * c.tile_w, c.dim);
+  const double w = (double)std::min<int64_t>(c.tile_w, c.dim - tile * c.tile_w);
+  const int64_t it = std::min<int64_t>(g * c.rpw, c.n_items);
+  return full * cols_before + (tile * c.tile_w < c.dim ? w * (double)P[it] : 0.0);
+}
+
+// XCD boundaries at equal shares of the total estimated cost.
+static XcdBounds xcd_bounds(const std::vector<SegCost> &costs, int64_t total_tasks) {
+  XcdBounds xb{};
+  std::vector<double> before(costs.size() + 1, 0.0);
+  for (size_t i = 0; i < costs.size(); ++i)
+    before[i + 1] = before[i] + seg_cost_before(costs[i], costs[i].task_end - costs[i].task_begin);
+  const double total = before.back();
+  xb.v[0] = 0;
+  xb.v[8] = (int32_t)total_tasks;
+  size_t si = 0;
+  for (int x = 1; x < 8; ++x) {
+    const double target = total * x / 8.0;
+    while (si + 1 < costs.size() && before[si + 1] <= target) ++si;
+    int64_t t = total_tasks;
+    if (!costs.empty() && total > 0) {
+      const SegCost &c = costs[si];
+      int64_t lo = 0, hi = c.task_end - c.task_begin;  // first k with cost(k) >= target - before
+      const double need = target - before[si];
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (seg_cost_before(c, mid) >= need) hi = mid;
+        else lo = mid + 1;
+      }
+      t = c.task_begin + lo;
+    }
+    xb.v[x] = (int32_t)std::max<int64_t>(xb.v[x - 1], std::min<int64_t>(t, total_tasks));
+  }
+  return xb;
 }
 
 int ilog2(int v) {
@@ -259,6 +232,7 @@ struct cola_batch_s {
   // device tables
   BatchSeg *d_segs = nullptr;
   int n_segs = 0, total_tasks = 0, seg_cap = 0;
+  XcdBounds xb{};            // task range of each XCD
   HubSeg *d_hubs = nullptr;
   int n_hubs = 0, hub_cap = 0;
   int64_t hub_threads = 0;
@@ -277,8 +251,8 @@ static void drop_graph(cola_batch_s *b) {
 // Appends one problem's segments (degree bins cut into tasks) and hub table
 // entry; task / hub-thread counters run across the whole work list.
 static cola_status append_problem(const cola_batch_problem &pr, const cola_batch_tuning &t, float *partials,
-                                  int dtype, int flags, std::vector<BatchSeg> *segs, std::vector<HubSeg> *hubs,
-                                  int64_t *task, int64_t *hthreads) {
+                                  int dtype, int flags, std::vector<BatchSeg> *segs, std::vector<SegCost> *costs,
+                                  std::vector<HubSeg> *hubs, int64_t *task, int64_t *hthreads) {
   const cola_plan_s *p = pr.plan;
   const int ev = vec_elems(dtype);
   int tile = pr.dim;
@@ -318,12 +292,14 @@ static cola_status append_problem(const cola_batch_problem &pr, const cola_batch
     s.n_tiles = n_tiles;
     s.tile_w = tile;
     s.g_log2 = ilog2(G);
-    const int64_t tasks = (bin->n + s.rpw - 1) / s.rpw * n_tiles;
+    s.n_grp = (int32_t)((bin->n + s.rpw - 1) / s.rpw);
+    const int64_t tasks = (int64_t)s.n_grp * n_tiles;
     s.task_begin = (int32_t)*task;
     *task += tasks;
-    if (*task > INT32_MAX) return fail(COLA_ERR_INVALID, "batch: too many tasks");
+    if (*task > INT32_MAX - (1 << 24)) return fail(COLA_ERR_INVALID, "batch: too many tasks");
     s.task_end = (int32_t)*task;
     segs->push_back(s);
+    costs->push_back(SegCost{&bin->cost, s.task_begin, s.task_end, s.n_grp, s.rpw, s.tile_w, s.dim, s.n_items});
   }
   if (p->num_hubs > 0) {
     HubSeg h{};
@@ -343,19 +319,19 @@ static cola_status append_problem(const cola_batch_problem &pr, const cola_batch
 }
 
 // Launches the persistent row kernel over a work list and the hub reduction.
-static cola_status launch_tables(const BatchSeg *d_segs, int n_segs, int total_tasks, const HubSeg *d_hubs,
-                                 int n_hubs, int64_t hub_threads, int dtype, int flags, int blocks_per_cu,
-                                 hipStream_t s) {
+static cola_status launch_tables(const BatchSeg *d_segs, int n_segs, int total_tasks, const XcdBounds &xb,
+                                 const HubSeg *d_hubs, int n_hubs, int64_t hub_threads, int dtype, int flags,
+                                 int blocks_per_cu, hipStream_t s) {
   const int accumulate = (flags & COLA_OVERWRITE) ? 0 : 1;
   const bool kahan = (flags & COLA_KAHAN) != 0;
   if (total_tasks > 0) {
     int64_t blocks = std::min<int64_t>(((int64_t)total_tasks + 3) / 4, (int64_t)device_cu_count() * blocks_per_cu);
     blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
     switch (dtype) {
-      case COLA_F32: launch_batch<COLA_F32, 4>(d_segs, n_segs, total_tasks, accumulate, kahan, (int)blocks, s); break;
-      case COLA_BF16: launch_batch<COLA_BF16, 8>(d_segs, n_segs, total_tasks, accumulate, kahan, (int)blocks, s); break;
-      case COLA_F16: launch_batch<COLA_F16, 8>(d_segs, n_segs, total_tasks, accumulate, kahan, (int)blocks, s); break;
-      case COLA_FP8_E4M3: launch_batch<COLA_FP8_E4M3, 16>(d_segs, n_segs, total_tasks, accumulate, kahan, (int)blocks, s); break;
+      case COLA_F32: launch_batch<COLA_F32, 4>(d_segs, n_segs, xb, accumulate, kahan, (int)blocks, s); break;
+      case COLA_BF16: launch_batch<COLA_BF16, 8>(d_segs, n_segs, xb, accumulate, kahan, (int)blocks, s); break;
+      case COLA_F16: launch_batch<COLA_F16, 8>(d_segs, n_segs, xb, accumulate, kahan, (int)blocks, s); break;
+      case COLA_FP8_E4M3: launch_batch<COLA_FP8_E4M3, 16>(d_segs, n_segs, xb, accumulate, kahan, (int)blocks, s); break;
     }
   }
   if (hub_threads > 0) {
@@ -370,13 +346,15 @@ static cola_status launch_tables(const BatchSeg *d_segs, int n_segs, int total_t
 static cola_status build_tables(cola_batch_s *b) {
   drop_graph(b);
   std::vector<BatchSeg> segs;
+  std::vector<SegCost> costs;
   std::vector<HubSeg> hubs;
   int64_t task = 0, hthreads = 0;
   for (size_t k = 0; k < b->probs.size(); ++k) {
-    cola_status st = append_problem(b->probs[k], b->tune[k], b->partials[k], b->dtype, b->flags, &segs, &hubs,
-                                    &task, &hthreads);
+    cola_status st = append_problem(b->probs[k], b->tune[k], b->partials[k], b->dtype, b->flags, &segs, &costs,
+                                    &hubs, &task, &hthreads);
     if (st) return st;
   }
+  b->xb = xcd_bounds(costs, task);
   if ((int)segs.size() > b->seg_cap) {
     (void)hipFree(b->d_segs);
     b->d_segs = nullptr;
@@ -412,8 +390,8 @@ static cola_status build_tables(cola_batch_s *b) {
 }
 
 static cola_status run_batch(cola_batch_s *b, hipStream_t s) {
-  return launch_tables(b->d_segs, b->n_segs, b->total_tasks, b->d_hubs, b->n_hubs, b->hub_threads, b->dtype, b->flags,
-                       b->blocks_per_cu, s);
+  return launch_tables(b->d_segs, b->n_segs, b->total_tasks, b->xb, b->d_hubs, b->n_hubs, b->hub_threads, b->dtype,
+                       b->flags, b->blocks_per_cu, s);
 }
 
 static void free_batch(cola_batch_s *b) {
@@ -593,7 +571,7 @@ cola_status cola_batch_autotune(cola_batch b, int iters, void *stream, float *be
         t.rpw_short = r;
         cands.push_back(t);
       }
-      for (int r : {1, 2, 4}) {
+      for (int r : {1, 2, 4, 8}) {
         cola_batch_tuning t = cur;
         t.rpw_long = r;
         cands.push_back(t);
@@ -618,7 +596,7 @@ cola_status cola_batch_autotune(cola_batch b, int iters, void *stream, float *be
     if (st) break;
     const int bpc0 = b->blocks_per_cu;
     int best_bpc = bpc0;
-    for (int bpc : {2, 4, 8, 16}) {
+    for (int bpc : {1, 2, 3, 4, 8, 16}) {
       b->blocks_per_cu = bpc;
       float ms = 0.f;
       if ((st = measure(&ms))) break;
@@ -857,6 +835,7 @@ cola_status cola_server_run_trace(cola_server sv, int64_t n, const double *arriv
   };
   auto launch = [&](LaunchSlot &sl, const std::vector<int64_t> &batch, double tnow) -> cola_status {
     std::vector<BatchSeg> segs;
+    std::vector<SegCost> costs;
     std::vector<HubSeg> hubs;
     int64_t task = 0, hth = 0;
     for (int64_t i : batch) {
@@ -866,7 +845,7 @@ cola_status cola_server_run_trace(cola_server sv, int64_t n, const double *arriv
       pr.ld_C = pr.dim;
       auto it = sv->tune.find(pr.dim);
       const cola_batch_tuning t = it != sv->tune.end() ? it->second : cola_batch_tuning{0, 0, 1};
-      cola_status e = append_problem(pr, t, rs[i].partials, sv->o.dtype, flags, &segs, &hubs, &task, &hth);
+      cola_status e = append_problem(pr, t, rs[i].partials, sv->o.dtype, flags, &segs, &costs, &hubs, &task, &hth);
       if (e) return e;
       records[i].dispatch_us = tnow;
       records[i].batch_size = (int32_t)batch.size();
@@ -882,8 +861,8 @@ cola_status cola_server_run_trace(cola_server sv, int64_t n, const double *arriv
     if (!hubs.empty() && (e = hip_check(hipMemcpyAsync(sl.d_hubs, sl.h_hubs, hubs.size() * sizeof(HubSeg),
                                                        hipMemcpyHostToDevice, sl.stream), "hipMemcpyAsync")))
       return e;
-    if ((e = launch_tables(sl.d_segs, (int)segs.size(), (int)task, sl.d_hubs, (int)hubs.size(), hth, sv->o.dtype,
-                           flags, 8, sl.stream)))
+    if ((e = launch_tables(sl.d_segs, (int)segs.size(), (int)task, xcd_bounds(costs, task), sl.d_hubs,
+                           (int)hubs.size(), hth, sv->o.dtype, flags, 8, sl.stream)))
       return e;
     if ((e = hip_check(hipEventRecord(sl.done, sl.stream), "hipEventRecord"))) return e;
     sl.reqs = batch;

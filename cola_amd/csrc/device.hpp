@@ -38,6 +38,9 @@ struct PlanBin {
   int4 *items = nullptr;
   int32_t *xcd = nullptr;  // 9 entries: item range per XCD
   int64_t n = 0;
+  // Host: prefix sums of each item's cost (entries + 8), n + 1 values; the
+  // batched form splits its work list over the XCDs with it.
+  std::vector<int64_t> cost;
 };
 
 struct cola_plan_s {

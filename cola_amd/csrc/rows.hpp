@@ -141,8 +141,12 @@ struct RowArgs {
                   // only, added to what phase 1 wrote
 };
 
+// One step of a wave: the `rpw` items it0 .. it0+rpw-1 (those below ie), one
+// per team, over the column range [0, a.dim) of the tile a's pointers start
+// at.  Shared by the plan kernel (compile-time G per launch) and the batched
+// persistent kernel (G chosen per segment).
 template <int DT, int EPL, int G, int NV, int U, bool KAHAN>
-__global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
+__device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int rpw) {
   using V = Vec<DT, EPL>;
   using T = typename V::T;
   using Raw = typename V::Raw;
@@ -151,11 +155,172 @@ __global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
   const T *__restrict__ Bp = static_cast<const T *>(a.Bp);
   const int lane = threadIdx.x & 63;
   const int sub = lane / G, gl = lane % G;
-  const int rpw = a.rpw;
   const int gpt = S / rpw;     // groups per team
   const int tsize = 64 / rpw;  // lanes per team
   const int team = sub / gpt, tsub = sub % gpt;
   const int tbase = team * tsize, tl = lane - tbase;
+  const int it = it0 + team;
+  const bool has = it < ie;
+  int4 item = make_int4(0, 0, 0, 0);
+  if (has) item = a.items[it];
+  int b = item.y, e = item.z;
+  if (a.phase == 1) e = b + (item.w >> 1);
+  else if (a.phase == 2) b += item.w >> 1;
+  // Compensated sums only in the KAHAN instantiation and on flagged items.
+  const bool comp = KAHAN && (item.w & 1);
+  float acc[NV][EPL], err[NV][EPL];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) acc[k][q] = err[k][q] = 0.f;
+
+  // Entries are taken in chunks of K*tsize: each team lane stages K
+  // (column, value) pairs with coalesced loads, the next chunk prefetched
+  // while this one is used.  A lane whose entry lies past the item's end
+  // stages the last entry's column with value 0, so every broadcast slot
+  // holds a valid row address and no load or FMA is predicated.
+  constexpr int K = U > G ? U / G : 1;  // pairs per lane: one batch of U per group fills K*tsize
+  const int csize = K * tsize;
+  int mc[K], mc_next[K];
+  float mv[K], mv_next[K];
+  auto stage = [&](int base, int (&c)[K], float (&v)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int idx = base + k * tsize + tl;
+      const int cl = min(idx, e - 1);
+      c[k] = a.cols[cl];
+      const float x = a.vals[cl];
+      v[k] = idx < e ? x : 0.f;
+    }
+  };
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    mc[k] = mc_next[k] = 0;
+    mv[k] = mv_next[k] = 0.f;
+  }
+  if (b < e) stage(b, mc, mv);
+  for (int base = b; base < e; base += csize) {
+    const int n = min(csize, e - base);
+    if (base + csize < e) stage(base + csize, mc_next, mv_next);
+    for (int j = 0; j < n; j += gpt * U) {
+      // Broadcast the batch's (column, value) pairs first, then issue every
+      // load.  With K > 1 the batch is the whole chunk (j = 0) and slot u
+      // reads pair u / G of team lane (u % G) * gpt + tsub.
+      int cu[U];
+      float vu[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = K > 1 ? u / G : 0;
+        const int src = (tbase + (K > 1 ? (u % G) * gpt + tsub : j + u * gpt + tsub)) << 2;
+        cu[u] = __builtin_amdgcn_ds_bpermute(src, mc[k]);
+        vu[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv[k])));
+      }
+      Raw raw[U][NV];
+      float av[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int c = cu[u];
+        const bool loc = c >= 0;
+        const T *rp = (loc ? Bl : Bp) + (int64_t)(loc ? c : ~c) * (loc ? a.ldB : a.ldP);
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          const int col = (k * G + gl) * EPL;
+          raw[u][k] = V::load(rp + (col < a.dim ? col : 0));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        float v = vu[u];
+        if constexpr (DT == COLA_FP8_E4M3) v *= (cu[u] >= 0 ? a.sl[cu[u]] : a.sp[~cu[u]]);
+        av[u] = v;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+          float x[EPL];
+          V::widen(raw[u][k], x);
+          if (!comp) {
+#pragma unroll
+            for (int q = 0; q < EPL; ++q) acc[k][q] = fmaf(av[u], x[q], acc[k][q]);
+          } else {
+            // Kahan: err carries the low-order part lost by acc.
+#pragma unroll
+            for (int q = 0; q < EPL; ++q) {
+              const float y = fmaf(av[u], x[q], -err[k][q]);
+              const float t = acc[k][q] + y;
+              err[k][q] = (t - acc[k][q]) - y;
+              acc[k][q] = t;
+            }
+          }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      mc[k] = mc_next[k];
+      mv[k] = mv_next[k];
+    }
+  }
+  // Sum the groups of each team (xor offsets below tsize stay in the team).
+  for (int off = G; off < tsize; off <<= 1) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) {
+        const float s2 = __shfl_xor(acc[k][q], off);
+        const float e2 = __shfl_xor(err[k][q], off);
+        if (!comp) {
+          acc[k][q] += s2;
+        } else {
+          // Neumaier combination of (acc, -err) pairs.
+          const float s1 = acc[k][q];
+          const float t = s1 + s2;
+          const float lost = fabsf(s1) >= fabsf(s2) ? (s1 - t) + s2 : (s2 - t) + s1;
+          acc[k][q] = t;
+          err[k][q] = err[k][q] + e2 - lost;
+        }
+      }
+  }
+  if (has && tsub == 0) {
+    float *dst;
+    bool add_old;
+    if (item.x >= 0) {
+      dst = a.C + (int64_t)item.x * a.ldC;
+      add_old = a.accumulate != 0 || a.phase == 2;
+    } else {
+      dst = a.partials + (int64_t)(-item.x - 1) * a.pstride;
+      add_old = a.phase == 2;
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const int col = (k * G + gl) * EPL;
+      if (col < a.dim) {
+        float v[EPL];
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) v[q] = comp ? acc[k][q] - err[k][q] : acc[k][q];
+        if constexpr (EPL % 4 == 0) {
+#pragma unroll
+          for (int q = 0; q < EPL; q += 4) {
+            float4 *d4 = reinterpret_cast<float4 *>(dst + col + q);
+            float4 o = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
+            if (add_old) {
+              const float4 old = *d4;
+              o.x += old.x; o.y += old.y; o.z += old.z; o.w += old.w;
+            }
+            *d4 = o;
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < EPL; ++q) dst[col + q] = add_old ? dst[col + q] + v[q] : v[q];
+        }
+      }
+    }
+  }
+}
+
+template <int DT, int EPL, int G, int NV, int U, bool KAHAN>
+__global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
+  const int rpw = a.rpw;
   const int wpb = blockDim.x >> 6;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int blk, nblk, ib, ie;
@@ -173,166 +338,7 @@ __global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
   }
   const int wave = blk * wpb + wid;
   const int nwave = nblk * wpb;
-
-  for (int it0 = ib + wave * rpw; it0 < ie; it0 += nwave * rpw) {
-    const int it = it0 + team;
-    const bool has = it < ie;
-    int4 item = make_int4(0, 0, 0, 0);
-    if (has) item = a.items[it];
-    int b = item.y, e = item.z;
-    if (a.phase == 1) e = b + (item.w >> 1);
-    else if (a.phase == 2) b += item.w >> 1;
-    // Compensated sums only in the KAHAN instantiation and on flagged items.
-    const bool comp = KAHAN && (item.w & 1);
-    float acc[NV][EPL], err[NV][EPL];
-#pragma unroll
-    for (int k = 0; k < NV; ++k)
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) acc[k][q] = err[k][q] = 0.f;
-
-    // Entries are taken in chunks of K*tsize: each team lane stages K
-    // (column, value) pairs with coalesced loads, the next chunk prefetched
-    // while this one is used.  A lane whose entry lies past the item's end
-    // stages the last entry's column with value 0, so every broadcast slot
-    // holds a valid row address and no load or FMA is predicated.
-    constexpr int K = U > G ? U / G : 1;  // pairs per lane: one batch of U per group fills K*tsize
-    const int csize = K * tsize;
-    int mc[K], mc_next[K];
-    float mv[K], mv_next[K];
-    auto stage = [&](int base, int (&c)[K], float (&v)[K]) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const int idx = base + k * tsize + tl;
-        const int cl = min(idx, e - 1);
-        c[k] = a.cols[cl];
-        const float x = a.vals[cl];
-        v[k] = idx < e ? x : 0.f;
-      }
-    };
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      mc[k] = mc_next[k] = 0;
-      mv[k] = mv_next[k] = 0.f;
-    }
-    if (b < e) stage(b, mc, mv);
-    for (int base = b; base < e; base += csize) {
-      const int n = min(csize, e - base);
-      if (base + csize < e) stage(base + csize, mc_next, mv_next);
-      for (int j = 0; j < n; j += gpt * U) {
-        // Broadcast the batch's (column, value) pairs first, then issue every
-        // load.  With K > 1 the batch is the whole chunk (j = 0) and slot u
-        // reads pair u / G of team lane (u % G) * gpt + tsub.
-        int cu[U];
-        float vu[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int k = K > 1 ? u / G : 0;
-          const int src = (tbase + (K > 1 ? (u % G) * gpt + tsub : j + u * gpt + tsub)) << 2;
-          cu[u] = __builtin_amdgcn_ds_bpermute(src, mc[k]);
-          vu[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv[k])));
-        }
-        Raw raw[U][NV];
-        float av[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int c = cu[u];
-          const bool loc = c >= 0;
-          const T *rp = (loc ? Bl : Bp) + (int64_t)(loc ? c : ~c) * (loc ? a.ldB : a.ldP);
-#pragma unroll
-          for (int k = 0; k < NV; ++k) {
-            const int col = (k * G + gl) * EPL;
-            raw[u][k] = V::load(rp + (col < a.dim ? col : 0));
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          float v = vu[u];
-          if constexpr (DT == COLA_FP8_E4M3) v *= (cu[u] >= 0 ? a.sl[cu[u]] : a.sp[~cu[u]]);
-          av[u] = v;
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-#pragma unroll
-          for (int k = 0; k < NV; ++k) {
-            float x[EPL];
-            V::widen(raw[u][k], x);
-            if (!comp) {
-#pragma unroll
-              for (int q = 0; q < EPL; ++q) acc[k][q] = fmaf(av[u], x[q], acc[k][q]);
-            } else {
-              // Kahan: err carries the low-order part lost by acc.
-#pragma unroll
-              for (int q = 0; q < EPL; ++q) {
-                const float y = fmaf(av[u], x[q], -err[k][q]);
-                const float t = acc[k][q] + y;
-                err[k][q] = (t - acc[k][q]) - y;
-                acc[k][q] = t;
-              }
-            }
-          }
-      }
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        mc[k] = mc_next[k];
-        mv[k] = mv_next[k];
-      }
-    }
-    // Sum the groups of each team (xor offsets below tsize stay in the team).
-    for (int off = G; off < tsize; off <<= 1) {
-#pragma unroll
-      for (int k = 0; k < NV; ++k)
-#pragma unroll
-        for (int q = 0; q < EPL; ++q) {
-          const float s2 = __shfl_xor(acc[k][q], off);
-          const float e2 = __shfl_xor(err[k][q], off);
-          if (!comp) {
-            acc[k][q] += s2;
-          } else {
-            // Neumaier combination of (acc, -err) pairs.
-            const float s1 = acc[k][q];
-            const float t = s1 + s2;
-            const float lost = fabsf(s1) >= fabsf(s2) ? (s1 - t) + s2 : (s2 - t) + s1;
-            acc[k][q] = t;
-            err[k][q] = err[k][q] + e2 - lost;
-          }
-        }
-    }
-    if (has && tsub == 0) {
-      float *dst;
-      bool add_old;
-      if (item.x >= 0) {
-        dst = a.C + (int64_t)item.x * a.ldC;
-        add_old = a.accumulate != 0 || a.phase == 2;
-      } else {
-        dst = a.partials + (int64_t)(-item.x - 1) * a.pstride;
-        add_old = a.phase == 2;
-      }
-#pragma unroll
-      for (int k = 0; k < NV; ++k) {
-        const int col = (k * G + gl) * EPL;
-        if (col < a.dim) {
-          float v[EPL];
-#pragma unroll
-          for (int q = 0; q < EPL; ++q) v[q] = comp ? acc[k][q] - err[k][q] : acc[k][q];
-          if constexpr (EPL % 4 == 0) {
-#pragma unroll
-            for (int q = 0; q < EPL; q += 4) {
-              float4 *d4 = reinterpret_cast<float4 *>(dst + col + q);
-              float4 o = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
-              if (add_old) {
-                const float4 old = *d4;
-                o.x += old.x; o.y += old.y; o.z += old.z; o.w += old.w;
-              }
-              *d4 = o;
-            }
-          } else {
-#pragma unroll
-            for (int q = 0; q < EPL; ++q) dst[col + q] = add_old ? dst[col + q] + v[q] : v[q];
-          }
-        }
-      }
-    }
-  }
+  for (int it0 = ib + wave * rpw; it0 < ie; it0 += nwave * rpw) row_step<DT, EPL, G, NV, U, KAHAN>(a, it0, ie, rpw);
 }
 
 // Lanes per row group and vectors per lane for a dim.  vec_ok: dim, leading

@@ -811,9 +811,17 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
     free_plan(p);
     return st;
   }
+  auto cost_prefix = [](const std::vector<int4> &v) {
+    std::vector<int64_t> c(v.size() + 1, 0);
+    for (size_t i = 0; i < v.size(); ++i) c[i + 1] = c[i] + (v[i].z - v[i].y) + 8;
+    return c;
+  };
+  p->shortb.cost = cost_prefix(is);
+  p->longb.cost = cost_prefix(il);
   {
     std::vector<int4> ie(hp.empty_rows.size());
     for (size_t i = 0; i < ie.size(); ++i) ie[i] = make_int4(hp.empty_rows[i], 0, 0, 0);
+    p->emptyb.cost = cost_prefix(ie);
     p->emptyb.n = (int64_t)ie.size();
     if ((st = upload(ie, &p->emptyb.items))) {
       free_plan(p);
