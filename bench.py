@@ -172,27 +172,34 @@ def serving_section(cola, torch, log, args, dim_tunings):
     resident_bytes = sum(p["plan"].info["device_bytes"] + p["B"].numel() * 2 + p["Bf"].numel() * 4 for p in res)
     log(f"serving: {len(res)} resident graphs ({resident_bytes / 1e9:.1f} GB incl. FP32 copies), "
         f"built in {time.time() - t0:.1f}s")
-    arr, rid = workloads.poisson_trace(args.serve_rate, args.serve_seconds, len(res), seed=5)
     budget = 32 << 30
     out = {"resident_graphs": len(res), "rate_req_per_s": args.serve_rate, "trace_seconds": args.serve_seconds,
            "memory_budget_bytes": budget, "resident_bytes": resident_bytes}
-    sv = cola.Server(budget, dtype=cola.BF16, max_batch=64, max_inflight=2, kahan=True)
-    ids = [sv.add_resident(p["plan"], p["B"]) for p in res]
+    bpc = next(iter(dim_tunings.values()))["blocks_per_cu"] if dim_tunings else 0
+    sv = cola.Server(budget, dtype=cola.BF16, max_batch=64, max_inflight=2, kahan=True, blocks_per_cu=bpc)
+    ids_b = [sv.add_resident(p["plan"], p["B"]) for p in res]
     for d, t in dim_tunings.items():
         sv.set_tuning(d, col_tile=t["col_tile"], rpw_short=t["rpw_short"], rpw_long=t["rpw_long"])
-    sv.run_trace(arr[:50] - arr[0], rid[:50])  # warm-up
-    recs, wall, peak = sv.run_trace(arr, np.asarray(ids, np.int32)[rid])
-    out["continuous_batching_bf16"] = dict(workloads.latency_stats(recs, wall), peak_pool_bytes=peak)
-    del sv
     pl = cola.Server(budget, dtype=cola.F32, plain=True)
-    ids = [pl.add_resident(p["plan"], p["Bf"]) for p in res]
-    pl.run_trace(arr[:50] - arr[0], rid[:50])
-    recs, wall, peak = pl.run_trace(arr, np.asarray(ids, np.int32)[rid])
-    out["plain_per_request_f32"] = dict(workloads.latency_stats(recs, wall), peak_pool_bytes=peak)
-    b, p = out["continuous_batching_bf16"], out["plain_per_request_f32"]
-    log(f"serving: batched bf16 {b['req_per_s']:.0f} req/s p50 {b['p50_ms']:.2f} ms p99 {b['p99_ms']:.2f} ms "
-        f"(mean batch {b['mean_batch']:.1f}); plain f32 {p['req_per_s']:.0f} req/s p50 {p['p50_ms']:.2f} ms "
-        f"p99 {p['p99_ms']:.2f} ms")
+    ids_p = [pl.add_resident(p["plan"], p["Bf"]) for p in res]
+    # The configured rate, then an overload point where requests arrive
+    # faster than one launch per request can serve them.
+    for rate, seconds, key in ((args.serve_rate, args.serve_seconds, ""),
+                               (args.serve_stress_rate, 1.0, "_at_%d_req_per_s" % int(args.serve_stress_rate))):
+        if rate <= 0:
+            continue
+        arr, rid = workloads.poisson_trace(rate, seconds, len(res), seed=5)
+        sv.run_trace(arr[:50] - arr[0], np.asarray(ids_b, np.int32)[rid[:50]])  # warm-up
+        recs, wall, peak = sv.run_trace(arr, np.asarray(ids_b, np.int32)[rid])
+        out["continuous_batching_bf16" + key] = dict(workloads.latency_stats(recs, wall), peak_pool_bytes=peak)
+        pl.run_trace(arr[:50] - arr[0], np.asarray(ids_p, np.int32)[rid[:50]])
+        recs, wall, peak = pl.run_trace(arr, np.asarray(ids_p, np.int32)[rid])
+        out["plain_per_request_f32" + key] = dict(workloads.latency_stats(recs, wall), peak_pool_bytes=peak)
+        b, p = out["continuous_batching_bf16" + key], out["plain_per_request_f32" + key]
+        log(f"serving at {rate:.0f} req/s: batched bf16 {b['req_per_s']:.0f} req/s p50 {b['p50_ms']:.2f} ms "
+            f"p99 {b['p99_ms']:.2f} ms (mean batch {b['mean_batch']:.1f}); plain f32 {p['req_per_s']:.0f} req/s "
+            f"p50 {p['p50_ms']:.2f} ms p99 {p['p99_ms']:.2f} ms")
+    del sv, pl
     return out
 
 
@@ -246,6 +253,8 @@ def main():
     ap.add_argument("--serve-graphs", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-stress-rate", type=float, default=20000.0,
+                    help="second serving point, 1 s of arrivals at this rate (0: skip)")
     ap.add_argument("--tuning", default="profile",
                     help='"profile" (the schedule of the committed counter profile when it was taken on this '
                          'workload, else autotune), "auto" (autotune before timing), a JSON object of plan '

@@ -73,7 +73,7 @@ class BatchTuning(ctypes.Structure):
 
 class ServerOpts(ctypes.Structure):
     _fields_ = [("memory_budget", c_i64), ("max_batch", c_i32), ("max_inflight", c_i32), ("dtype", c_i32),
-                ("flags", c_i32), ("plain", c_i32)]
+                ("flags", c_i32), ("plain", c_i32), ("blocks_per_cu", c_i32)]
 
 
 class PlanInfo(ctypes.Structure):

@@ -728,7 +728,9 @@ cola_status cola_server_create(const cola_server_opts *opts, cola_server *out) {
   if (o.memory_budget <= 0) return fail(COLA_ERR_INVALID, "server_create: memory_budget must be positive");
   if (o.max_batch <= 0) o.max_batch = 64;
   if (o.max_inflight <= 0) o.max_inflight = 2;
-  if (o.max_inflight > 16 || o.max_batch > 4096) return fail(COLA_ERR_INVALID, "server_create: limits too large");
+  if (o.blocks_per_cu <= 0) o.blocks_per_cu = 8;
+  if (o.max_inflight > 16 || o.max_batch > 4096 || o.blocks_per_cu > 64)
+    return fail(COLA_ERR_INVALID, "server_create: limits too large");
   if (o.flags & ~COLA_KAHAN) return fail(COLA_ERR_INVALID, "server_create: unknown flag");
   auto *sv = new (std::nothrow) cola_server_s;
   if (!sv) return fail(COLA_ERR_NOMEM, "server_create: out of host memory");
@@ -862,7 +864,7 @@ cola_status cola_server_run_trace(cola_server sv, int64_t n, const double *arriv
                                                        hipMemcpyHostToDevice, sl.stream), "hipMemcpyAsync")))
       return e;
     if ((e = launch_tables(sl.d_segs, (int)segs.size(), (int)task, xcd_bounds(costs, task), sl.d_hubs,
-                           (int)hubs.size(), hth, sv->o.dtype, flags, 8, sl.stream)))
+                           (int)hubs.size(), hth, sv->o.dtype, flags, sv->o.blocks_per_cu, sl.stream)))
       return e;
     if ((e = hip_check(hipEventRecord(sl.done, sl.stream), "hipEventRecord"))) return e;
     sl.reqs = batch;

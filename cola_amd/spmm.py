@@ -277,9 +277,10 @@ class Server:
     """Continuous-batching SpMM server over resident graphs (see
     cola_server_run_trace in include/cola_spmm.h)."""
 
-    def __init__(self, memory_budget, dtype=BF16, max_batch=64, max_inflight=2, kahan=False, plain=False):
+    def __init__(self, memory_budget, dtype=BF16, max_batch=64, max_inflight=2, kahan=False, plain=False,
+                 blocks_per_cu=0):
         o = _lib.ServerOpts(int(memory_budget), int(max_batch), int(max_inflight), int(dtype),
-                            KAHAN if kahan else 0, 1 if plain else 0)
+                            KAHAN if kahan else 0, 1 if plain else 0, int(blocks_per_cu))
         h = ctypes.c_void_p()
         check(lib().cola_server_create(ctypes.byref(o), ctypes.byref(h)))
         self._h = h

@@ -416,6 +416,8 @@ typedef struct {
   int32_t flags;         /* COLA_KAHAN or 0 (outputs are overwritten)      */
   int32_t plain;         /* 1: no batching, every request is its own launch
                             queued on one stream as it arrives (baseline)  */
+  int32_t blocks_per_cu; /* grid of each launch (0 = 8); e.g. the value
+                            cola_batch_get_tuning reports after autotuning */
 } cola_server_opts;
 typedef struct {
   double arrival_us, dispatch_us, done_us; /* from the start of the trace  */
