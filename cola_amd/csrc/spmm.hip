@@ -370,8 +370,11 @@ __global__ __launch_bounds__(256) void error_norms_kernel(const float *__restric
 // ---------------------------------------------------------------------------
 // Handles.
 // ---------------------------------------------------------------------------
+static void free_plan(cola_plan_s *p);
+
 static void free_zip(cola_zip_s *z) {
   if (!z) return;
+  free_plan(z->fast_plan);
   (void)hipFree(z->ptr);
   (void)hipFree(z->idx);
   (void)hipFree(z->zip);
@@ -590,6 +593,9 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
 
 }  // namespace
 
+static cola_status plan_from_partition(const cola::Partition &part, const cola_plan_opts &o, cola_plan_s **out);
+static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int peid, int dim, cola_plan_s **out);
+
 // ---------------------------------------------------------------------------
 // C ABI (device part).
 // ---------------------------------------------------------------------------
@@ -753,18 +759,24 @@ cola_status cola_dist_spmm_ppl_batch(float *output, const float *B_global, float
                                      cola_zip csr_zip_l, cola_zip csc_zip_r, int dim,
                                      int nodesPerPE, int peid, void *stream) {
   if (!csr_zip_l || !csc_zip_r) return fail(COLA_ERR_INVALID, "dist_spmm_ppl_batch: NULL handle");
-  if (!B_global || dim < 1 || nodesPerPE < 0 || peid < 0)
+  if (!B_global || dim < 1 || nodesPerPE < 1 || peid < 0)
     return fail(COLA_ERR_INVALID, "dist_spmm_ppl_batch: bad arguments");
   const int64_t L = csr_zip_l->nptr > 0 ? csr_zip_l->nptr - 1 : 0;
   const int64_t R = csc_zip_r->nptr > 0 ? csc_zip_r->nptr - 1 : 0;
   if (L > INT32_MAX || R > INT32_MAX) return fail(COLA_ERR_INVALID, "dist_spmm_ppl_batch: too many rows");
-  cola_status st = cola_pack_remote(B_global, dim, nullptr, csc_zip_r->uniq, csc_zip_r->n_unique, dim,
-                                    COLA_F32, packed, dim, nullptr, stream);
+  // The same sums as ppl0_batch_SpMM over the virtual rows / columns, run
+  // through a plan rebuilt from the two zips on the first call (cached in the
+  // local zip): no atomics, deterministic order.
+  cola_plan_s *plan = nullptr;
+  cola_status st = zip_pair_plan(csr_zip_l, csc_zip_r, nodesPerPE, peid, dim, &plan);
+  if (st) return st;
+  if (plan->U_r > 0 && !packed) return fail(COLA_ERR_INVALID, "dist_spmm_ppl_batch: packed is NULL");
+  st = cola_pack_remote(B_global, dim, nullptr, plan->remote_cols, plan->U_r, dim, COLA_F32, packed, dim, nullptr,
+                        stream);
   if (st) return st;
   const float *input = B_global + (int64_t)peid * nodesPerPE * dim;
-  return cola_ppl0_batch_spmm(output, input, packed, csr_zip_l->ptr, csr_zip_l->idx, csr_zip_l->zip,
-                              csr_zip_l->val, csc_zip_r->ptr, csc_zip_r->idx, csc_zip_r->pack_idx,
-                              csc_zip_r->val, 0, 0, (int)L, (int)R, dim, nodesPerPE, peid, stream);
+  return cola_spmm_partition(plan, output, dim, input, dim, nullptr, packed, dim, nullptr, dim, COLA_F32,
+                             COLA_ACCUMULATE, stream);
 }
 
 cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, cola_plan *out) {
@@ -775,8 +787,14 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
   if (o.hub_threshold < 0 || o.chunk_nnz < 0 || o.kahan_degree < 0 || o.max_dim < 0 || o.short_degree < 0 ||
       o.col_panels < 0 || o.col_panels > 64 || o.row_order < 0 || o.row_order > 1)
     return fail(COLA_ERR_INVALID, "plan_create: negative option");
+  return plan_from_partition(part->p, o, out);
+}
+
+}  // extern "C"
+
+static cola_status plan_from_partition(const cola::Partition &part, const cola_plan_opts &o, cola_plan_s **out) {
   cola::HostPlan hp;
-  cola_status st = cola::build_plan(part->p, o, &hp);
+  cola_status st = cola::build_plan(part, o, &hp);
   if (st) return st;
   auto *p = new (std::nothrow) cola_plan_s;
   if (!p) return fail(COLA_ERR_NOMEM, "plan_create: out of host memory");
@@ -856,6 +874,78 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
   *out = p;
   return COLA_OK;
 }
+
+template <typename T>
+static cola_status download(const T *d, int64_t n, std::vector<T> *h) {
+  h->resize((size_t)std::max<int64_t>(n, 0));
+  if (n <= 0) return COLA_OK;
+  return hip_check(hipMemcpy(h->data(), d, (size_t)n * sizeof(T), hipMemcpyDeviceToHost), "hipMemcpy zip arrays");
+}
+
+// Plan behind a (local zip, remote zip) pair, built on first use and cached in
+// the local zip while the pair, nodesPerPE, peid and a dim within the plan's
+// workspace stay the same.  The partition is recovered from the zips: the
+// virtual rows (columns) that nnz_split cut from one row (column) are
+// consecutive with the same id, so they are merged back; entries keep their
+// order.  Items in decreasing-degree order; schedule: 128-byte column tiles
+// for dims of 64 and more, 4 rows per wave on long rows, 2 blocks per CU, 16
+// loads in flight (the tuned schedule of the 2^22-node partition).
+static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int peid, int dim, cola_plan_s **out) {
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  if (l->fast_plan && l->fast_peer == r && l->fast_P == P && l->fast_peid == peid && dim <= l->fast_plan->max_dim) {
+    *out = l->fast_plan;
+    return COLA_OK;
+  }
+  free_plan(l->fast_plan);
+  l->fast_plan = nullptr;
+  std::vector<int32_t> lptr, lidx, lzip, rptr, ridx, rzip;
+  std::vector<float> lval, rval;
+  const int64_t Lv = l->nptr > 0 ? l->nptr - 1 : 0, Rv = r->nptr > 0 ? r->nptr - 1 : 0;
+  cola_status st;
+  if ((st = download(l->ptr, l->nptr, &lptr)) || (st = download(l->idx, l->nnz, &lidx)) ||
+      (st = download(l->zip, Lv, &lzip)) || (st = download(l->val, l->nnz, &lval)) ||
+      (st = download(r->ptr, r->nptr, &rptr)) || (st = download(r->idx, r->nnz, &ridx)) ||
+      (st = download(r->zip, Rv, &rzip)) || (st = download(r->val, r->nnz, &rval)))
+    return st;
+  cola::Partition part;
+  part.nodesPerPE = P;
+  part.rank = peid;
+  part.num_rows = P;
+  auto merge = [](const std::vector<int32_t> &ptr, const std::vector<int32_t> &zip, int64_t nv,
+                  std::vector<int32_t> *ids, std::vector<int32_t> *mptr) {
+    for (int64_t v = 0; v < nv; ++v)
+      if (ids->empty() || ids->back() != zip[v]) {
+        ids->push_back(zip[v]);
+        mptr->push_back(ptr[v]);
+      }
+    mptr->push_back(nv > 0 ? ptr[nv] : 0);
+  };
+  merge(lptr, lzip, Lv, &part.csr_rowIdx_l, &part.csr_rowPtr_l);
+  merge(rptr, rzip, Rv, &part.csc_colIdx_r, &part.csc_colPtr_r);
+  part.csr_colIdx_l = std::move(lidx);
+  part.csr_val_l = std::move(lval);
+  part.csc_rowIdx_r = std::move(ridx);
+  part.csc_val_r = std::move(rval);
+  const int64_t max_col = part.csc_colIdx_r.empty() ? 0 : part.csc_colIdx_r.back();
+  part.nGPU = (int32_t)std::max<int64_t>(peid + 1, max_col / P + 1);
+  part.num_nodes = (int64_t)part.nGPU * P;
+  if ((st = cola::validate_partition(part))) return st;
+  cola_plan_opts o{};
+  o.max_dim = std::max(dim, 256);
+  o.row_order = 1;
+  cola_plan_s *p = nullptr;
+  if ((st = plan_from_partition(part, o, &p))) return st;
+  p->tune = cola_plan_tuning{dim >= 64 ? 32 : 0, 0, 4, 2, 0, 0, 16, 0};
+  l->fast_plan = p;
+  l->fast_peer = r;
+  l->fast_P = P;
+  l->fast_peid = peid;
+  *out = p;
+  return COLA_OK;
+}
+
+extern "C" {
 
 cola_status cola_plan_get_info(cola_plan p, cola_plan_info *info) {
   if (!p || !info) return fail(COLA_ERR_INVALID, "plan_info: NULL argument");

@@ -257,9 +257,15 @@ cola_status cola_ppl0_batch_spmm(
     int nodesPerPE, int peid, void *stream);
 
 /* Single-stage distSpMM_ppl_batch_wrapper (dist_spmm_ppl_batch.cuh:851-918
- * with group_size == nranks <= 8, cuh:837-848): pack the unique remote rows
- * of `csc_zip_r` from B_global into `packed`, then cola_ppl0_batch_spmm over
- * every virtual row / column.  input = B_global + peid·nodesPerPE·dim. */
+ * with group_size == nranks <= 8, cuh:837-848): output += A_p·B, with the
+ * unique remote rows of `csc_zip_r` packed from B_global into `packed`
+ * (n_unique x dim) and input = B_global + peid·nodesPerPE·dim.  The same sums
+ * as ppl0_batch_SpMM over the virtual rows / columns, computed through a
+ * degree-binned plan rebuilt from the two zips on the first call and cached
+ * in csr_zip_l (no atomics, deterministic order); cola_ppl0_batch_spmm is the
+ * literal virtual-row kernel.  Calls on one zip from several threads are
+ * serialised while the plan is built; concurrent launches race on its hub
+ * workspace. */
 cola_status cola_dist_spmm_ppl_batch(float *output, const float *B_global,
                                      float *packed, cola_zip csr_zip_l,
                                      cola_zip csc_zip_r, int dim,

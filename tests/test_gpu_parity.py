@@ -57,7 +57,10 @@ class Case:
         torch.cuda.synchronize()
         return C.cpu().numpy(), plan
 
-    def run_reference_path(self, split=True):
+    def run_reference_path(self, split=True, literal=False, init=0.0, calls=1):
+        """The reference API: zips (after nnz_split) + the single-stage wrapper,
+        which runs through a plan cached from the zips; literal=True runs the
+        ppl0_batch_SpMM restatement (virtual rows, FP32 atomics) instead."""
         a = self.part.arrays()
         if split:
             tl, pl, _ = cola.nnz_split(a["csr_rowIdx_l"], a["csr_rowPtr_l"], len(a["csr_colIdx_l"]))
@@ -68,8 +71,15 @@ class Case:
         zl = cola.d_csr_zip(pl, a["csr_colIdx_l"], tl, a["csr_val_l"])
         zr = cola.d_csc_zip(pr, a["csc_rowIdx_r"], tr, a["csc_val_r"])
         packed = torch.empty((max(zr.n_unique, 1), self.dim), dtype=torch.float32, device=self.dev)
-        C = torch.zeros((self.P, self.dim), dtype=torch.float32, device=self.dev)
-        cola.dist_spmm_ppl_batch(C, self.B, packed, zl, zr, self.dim, self.P, self.rank)
+        C = torch.full((self.P, self.dim), float(init), dtype=torch.float32, device=self.dev)
+        for _ in range(calls):
+            if literal:
+                uniq = torch.from_numpy(np.unique(np.asarray(tr, np.int32))).to(self.dev)
+                cola.pack_remote(self.B, uniq, packed, num_cols=zr.n_unique)
+                cola.ppl0_batch_SpMM(C, self.B_local(), packed, zl, zr, 0, 0, zl.num, zr.num, self.dim, self.P,
+                                     self.rank)
+            else:
+                cola.dist_spmm_ppl_batch(C, self.B, packed, zl, zr, self.dim, self.P, self.rank)
         torch.cuda.synchronize()
         return C.cpu().numpy()
 
@@ -89,26 +99,36 @@ def test_plan_path_c1(c1):
     assert np.all(C[c1.M:] == 0)
 
 
-def test_reference_path_c1(c1):
-    C = c1.run_reference_path(split=True)
+@pytest.mark.parametrize("literal", [False, True])
+def test_reference_path_c1(c1, literal):
+    C = c1.run_reference_path(split=True, literal=literal)
     err = rel_fro(C[:c1.M], c1.ref)
-    print(f"reference path C1 rel_fro={err:.3e}")
+    print(f"reference path (literal={literal}) C1 rel_fro={err:.3e}")
     assert err <= FP32_TOL
-    C2 = c1.run_reference_path(split=False)
+    C2 = c1.run_reference_path(split=False, literal=literal)
     assert rel_fro(C2[:c1.M], c1.ref) <= FP32_TOL
 
 
-def test_reference_path_matches_fp32_restatement(c1):
-    """The HIP reference path against the oracle's FP32 restatement of
-    ppl0_batch_SpMM_kernel on the same virtual rows/columns."""
+@pytest.mark.parametrize("literal", [False, True])
+def test_reference_path_matches_fp32_restatement(c1, literal):
+    """Both forms of the reference API against the oracle's FP32 restatement
+    of ppl0_batch_SpMM_kernel on the same virtual rows/columns."""
     a = c1.part.arrays()
     tl, pl, _ = cola.nnz_split(a["csr_rowIdx_l"], a["csr_rowPtr_l"], len(a["csr_colIdx_l"]))
     tr, pr, _ = cola.nnz_split(a["csc_colIdx_r"], a["csc_colPtr_r"], len(a["csc_rowIdx_r"]))
     Cr = np.zeros((c1.P, c1.dim), np.float32)
     orc.ppl0_f32(Cr, c1.Bh, c1.dim, c1.P, c1.rank, pl, a["csr_colIdx_l"], tl, a["csr_val_l"], pr,
                  a["csc_rowIdx_r"], tr, a["csc_val_r"])
-    C = c1.run_reference_path(split=True)
+    C = c1.run_reference_path(split=True, literal=literal)
     assert rel_fro(C, Cr.astype(np.float64)) <= FP32_TOL
+
+
+def test_reference_wrapper_accumulates_and_reuses_its_plan(c1):
+    """The wrapper adds into C on every call (cuh:141,196), through the plan
+    cached from the zips after the first call."""
+    C = c1.run_reference_path(split=True, init=1.5, calls=3)
+    assert rel_fro((C[:c1.M] - 1.5) / 3.0, c1.ref) <= FP32_TOL
+    assert np.all(C[c1.M:] == 1.5)
 
 
 def test_accumulate_semantics(c1):
@@ -133,7 +153,7 @@ def test_known_answer_all_ones():
         lb = rank * case.P
         deg = np.diff(case.rp)[lb:lb + case.M].astype(np.float32)
         sag = orc.sag_unweighted(lb, lb + case.M, case.rp, case.ci, case.Bh[:case.n])
-        for C in (case.run_plan()[0], case.run_reference_path()):
+        for C in (case.run_plan()[0], case.run_reference_path(), case.run_reference_path(literal=True)):
             assert orc.check_result(C[:case.M], sag) == 0
             assert np.array_equal(C[:case.M], np.repeat(deg[:, None], 32, axis=1))
 
@@ -145,8 +165,9 @@ def test_dims(dim):
     case = Case(g, 2, 1, dim, seed=dim)
     C, _ = case.run_plan(max_dim=max(dim, 256))
     assert rel_fro(C[:case.M], case.ref) <= FP32_TOL
-    Cr = case.run_reference_path()
-    assert rel_fro(Cr[:case.M], case.ref) <= FP32_TOL
+    for literal in (False, True):
+        Cr = case.run_reference_path(literal=literal)
+        assert rel_fro(Cr[:case.M], case.ref) <= FP32_TOL, literal
 
 
 @pytest.mark.parametrize("rank", [0, 1, 2, 3])
@@ -158,6 +179,7 @@ def test_every_rank_and_reverse_order(rank):
         pytest.skip("empty partition")
     assert rel_fro(case.run_plan()[0][:case.M], case.ref) <= FP32_TOL
     assert rel_fro(case.run_reference_path()[:case.M], case.ref) <= FP32_TOL
+    assert rel_fro(case.run_reference_path(literal=True)[:case.M], case.ref) <= FP32_TOL
 
 
 def test_identity_graph():
@@ -168,6 +190,7 @@ def test_identity_graph():
     C, _ = case.run_plan()
     assert np.array_equal(C[:case.M], case.Bh[case.P:case.P + case.M])
     assert np.array_equal(case.run_reference_path()[:case.M], case.Bh[case.P:case.P + case.M])
+    assert np.array_equal(case.run_reference_path(literal=True)[:case.M], case.Bh[case.P:case.P + case.M])
 
 
 def test_all_remote_partition():
@@ -184,6 +207,7 @@ def test_all_remote_partition():
     assert case.info["nnz_l"] == 0 and case.info["nnz_r"] > 0
     assert rel_fro(case.run_plan()[0][:case.M], case.ref) <= FP32_TOL
     assert rel_fro(case.run_reference_path()[:case.M], case.ref) <= FP32_TOL
+    assert rel_fro(case.run_reference_path(literal=True)[:case.M], case.ref) <= FP32_TOL
 
 
 def test_all_local_partition():
@@ -193,6 +217,7 @@ def test_all_local_partition():
     assert case.info["U_r"] == 0
     assert rel_fro(case.run_plan()[0][:case.M], case.ref) <= FP32_TOL
     assert rel_fro(case.run_reference_path()[:case.M], case.ref) <= FP32_TOL
+    assert rel_fro(case.run_reference_path(literal=True)[:case.M], case.ref) <= FP32_TOL
 
 
 def test_empty_partition():
