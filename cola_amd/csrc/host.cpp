@@ -133,8 +133,9 @@ void assign_values(Graph *g, bool gcn) {
 }
 
 // Cuthill-McKee order: components in order of their minimum-degree vertex,
-// breadth-first, neighbours visited by increasing (degree, id); isolated
-// vertices last.  Returns new->old.
+// breadth-first over out-edges, neighbours visited by increasing (degree,
+// id); vertices never reached (isolated ones) last.  Returns new->old, a
+// permutation of [0, n) for any graph, symmetric or not.
 std::vector<int32_t> cuthill_mckee(const Graph &g) {
   const int64_t n = g.n;
   std::vector<int32_t> deg(n);
@@ -168,8 +169,10 @@ std::vector<int32_t> cuthill_mckee(const Graph &g) {
       order.insert(order.end(), nb.begin(), nb.end());
     }
   }
+  // Vertices never reached: isolated ones, and (in a directed graph) those
+  // without out-edges whose in-neighbours were not visited.
   for (int64_t i = 0; i < n; ++i)
-    if (deg[i] == 0) order.push_back((int32_t)i);
+    if (!seen[i]) order.push_back((int32_t)i);
   return order;
 }
 

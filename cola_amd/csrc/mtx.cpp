@@ -414,6 +414,16 @@ cola_status cola_graph_reorder(cola_graph g, int method, int32_t *new2old, cola_
     default:
       return fail(COLA_ERR_INVALID, "graph_reorder: unknown method " + std::to_string(method));
   }
+  {
+    std::vector<uint8_t> hit(n, 0);
+    bool ok = (int64_t)order.size() == n;
+    for (size_t i = 0; ok && i < order.size(); ++i) {
+      const int32_t v = order[i];
+      ok = v >= 0 && v < n && !hit[v];
+      if (ok) hit[v] = 1;
+    }
+    if (!ok) return fail(COLA_ERR_INVALID, "graph_reorder: the order is not a permutation");
+  }
   auto *h = new (std::nothrow) cola_graph_s;
   if (!h) return fail(COLA_ERR_NOMEM, "graph_reorder: out of host memory");
   cola::relabel(G, order, &h->g);

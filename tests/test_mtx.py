@@ -325,3 +325,20 @@ def test_process_graph_and_reorder_mtx_flow(tmp_path):
                 assert np.array_equal(p[stem], ref[(stem, rank)]), (nGPU, rank, stem)
             # a locality order leaves few remote columns per partition
             assert len(p["csc_colIdx_r"]) <= 2 * 3 * 2
+
+
+@pytest.mark.parametrize("method", ["cm", "rcm", "degree"])
+def test_reorder_directed_graph_with_sinks(method):
+    """Vertices without out-edges that the breadth-first walk reaches appear
+    once in the order (a directed graph, unlike the symmetric R-MAT ones)."""
+    n = 6
+    # 0 -> 1 -> 2 (2 is a sink), 3 -> 2, 4 isolated, 5 -> 0
+    g = cola.Graph.from_coo(n, [0, 1, 3, 5], [1, 2, 2, 0], [1.0, 2.0, 3.0, 4.0])
+    h, perm = g.reorder(method)
+    assert sorted(perm.tolist()) == list(range(n))
+    rp, ci, va = g.arrays()
+    A = sp.csr_matrix((va, ci, rp), shape=(n, n))
+    P = A[perm][:, perm]
+    P.sort_indices()
+    rp2, ci2, va2 = h.arrays()
+    assert np.array_equal(rp2, P.indptr) and np.array_equal(ci2, P.indices) and np.array_equal(va2, P.data)
