@@ -165,6 +165,15 @@ class Partition:
                                       csc_rowIdx_r=_i32(csc_rowIdx_r), csc_colIdx_r=_i32(csc_colIdx_r),
                                       csr_val_l=None if csr_val_l is None else _f32(csr_val_l),
                                       csc_val_r=None if csc_val_r is None else _f32(csc_val_r)).items()}
+        # The C ABI takes the counts L, nnz_l, U_r, nnz_r and reads that many
+        # elements from each array: check every length against them first.
+        L, nl = len(arrs["csr_rowIdx_l"]), len(arrs["csr_colIdx_l"])
+        U, nr = len(arrs["csc_colIdx_r"]), len(arrs["csc_rowIdx_r"])
+        want = dict(csr_rowPtr_l=L + 1, csc_colPtr_r=U + 1, csr_val_l=nl, csc_val_r=nr)
+        for k, n in want.items():
+            if arrs[k] is not None and len(arrs[k]) != n:
+                raise _lib.ColaError(_lib.COLA_ERR_INVALID, f"partition_from_arrays: len({k}) = {len(arrs[k])}, "
+                                                            f"expected {n}")
         pa = _lib.PartitionArrays()
         keep = []
         for k in _FIELDS:
@@ -264,6 +273,8 @@ def nnz_split(rowIdx, rowPtr, nnz):
     entries.  Returns (target, ptr, gran); an empty rowIdx gives empty
     target and ptr (the reference produces nothing in that case)."""
     ri, rp = _i32(rowIdx), _i32(rowPtr)
+    if len(ri) and len(rp) != len(ri) + 1:
+        raise _lib.ColaError(_lib.COLA_ERR_INVALID, f"nnz_split: len(rowPtr) = {len(rp)}, expected {len(ri) + 1}")
     nv, gran = ctypes.c_int64(), ctypes.c_int32()
     check(lib().cola_nnz_split(_ptr(ri), len(ri), _ptr(rp), int(nnz), None, None, ctypes.byref(nv),
                                ctypes.byref(gran)))

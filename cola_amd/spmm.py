@@ -15,7 +15,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import ACCUMULATE, BF16, F16, F32, FP8_E4M3, KAHAN, OVERWRITE, check, lib  # noqa: F401
+from ._lib import (ACCUMULATE, BF16, COLA_ERR_INVALID, F16, F32, FP8_E4M3, KAHAN, OVERWRITE, ColaError,  # noqa: F401
+                   check, lib)
 
 DTYPE_BYTES = {F32: 4, BF16: 2, F16: 2, FP8_E4M3: 1}
 DTYPE_NAMES = {F32: "f32", BF16: "bf16", F16: "f16", FP8_E4M3: "fp8_e4m3"}
@@ -55,6 +56,10 @@ class _Zip:
     def __init__(self, ptr, idx, zip_ids, val=None):
         ptr, idx, zid = _host(ptr, np.int32), _host(idx, np.int32), _host(zip_ids, np.int32)
         va = None if val is None else _host(val, np.float32)
+        # The C ABI reads nptr-1 ids and nnz values: check the lengths here.
+        if len(zid) != max(len(ptr) - 1, 0) or (va is not None and len(va) != len(idx)):
+            raise ColaError(COLA_ERR_INVALID, f"zip: {len(ptr)} pointers need {max(len(ptr) - 1, 0)} ids "
+                                              f"(got {len(zid)}) and one value per index")
         h = ctypes.c_void_p()
         vp = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None and a.size else None  # noqa: E731
         check(lib().cola_zip_create(vp(ptr), len(ptr), vp(idx), len(idx), vp(zid), vp(va), ctypes.byref(h)))

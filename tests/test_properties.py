@@ -7,7 +7,11 @@ on random graphs drawn by hypothesis (host only, a few seconds).
   panel exactly;
 - MatrixMarket (prep_torch_par.py:14-20): random matrices written by scipy
   read back equal to scipy.io.mmread + tocsr(); the library's writer and
-  reader round-trip every value.
+  reader round-trip every value;
+- reorders keep the product up to the permutation;
+- malformed inputs to the loader, nnz_split and the zip constructors are
+  rejected with a status, and the plan builder runs on random partitions
+  and options without faulting.
 """
 import numpy as np
 import pytest
@@ -126,3 +130,71 @@ def test_reorder_preserves_the_product(method, g):
     AB = orc.spmm_f64(rp, ci, va, 0, n, B)
     HB = orc.spmm_f64(hp, hc, hv, 0, n, B[perm])
     assert np.allclose(HB, AB[perm], rtol=0, atol=1e-9)
+
+
+small_ints = st.lists(st.integers(-3, 12), min_size=0, max_size=7)
+
+
+@settings(**dict(SETTINGS, max_examples=300))
+@given(st.integers(-2, 24), st.integers(-1, 4), st.integers(-1, 4), small_ints, small_ints, small_ints, small_ints,
+       small_ints, small_ints)
+def test_partition_from_arrays_rejects_without_crashing(n, nGPU, rank, rp, cl, ri, cp, rr, cr):
+    """Arbitrary (mostly malformed) partition arrays: either a valid partition
+    or a ColaError, never a crash (the loader checks what the writer asserts
+    plus every index range)."""
+    try:
+        part = cola.Partition.from_arrays(n, nGPU, rank, rp, cl, ri, cp, rr, cr)
+    except cola.ColaError as e:
+        assert e.code == cola._lib.COLA_ERR_INVALID
+        return
+    a = part.arrays()
+    assert np.array_equal(a["csr_rowPtr_l"], np.asarray(rp, np.int32))
+    assert np.array_equal(a["csc_colIdx_r"], np.asarray(cr, np.int32))
+
+
+@settings(**dict(SETTINGS, max_examples=80))
+@given(graphs(max_n=400, max_deg=30), st.integers(1, 6), st.data())
+def test_host_plan_builder_on_random_partitions(g, nGPU, data):
+    """The plan builder (degree bins, hub chunks, XCD ranges, panels) runs on
+    random partitions and option sets; without a GPU the upload then fails
+    with a HIP status, with one it succeeds — never a crash."""
+    n, r, c, v = g
+    G = cola.Graph.from_coo(n, r, c, v)
+    rank = data.draw(st.integers(0, nGPU - 1))
+    part = G.partition(nGPU, rank)
+    opts = dict(hub_threshold=data.draw(st.sampled_from([0, 1, 4, 40])),
+                chunk_nnz=data.draw(st.sampled_from([0, 1, 3, 16])),
+                kahan_degree=data.draw(st.sampled_from([0, 2, 50])),
+                short_degree=data.draw(st.sampled_from([0, 1, 5])),
+                col_panels=data.draw(st.sampled_from([0, 2, 5])),
+                row_order=data.draw(st.sampled_from([0, 1])))
+    try:
+        plan = cola.Plan(part, **opts)
+    except cola.ColaError as e:
+        assert e.code == cola._lib.COLA_ERR_HIP, str(e)
+        return
+    assert plan.info["nnz"] == part.info()["nnz_l"] + part.info()["nnz_r"]
+
+
+@settings(**dict(SETTINGS, max_examples=200))
+@given(small_ints, small_ints, st.integers(-1, 40))
+def test_nnz_split_rejects_without_crashing(ri, rp, nnz):
+    """Malformed (rowIdx, rowPtr, nnz): a ColaError or the oracle's answer."""
+    try:
+        t, p, gran = cola.nnz_split(np.asarray(ri, np.int32), np.asarray(rp, np.int32), nnz)
+    except cola.ColaError as e:
+        assert e.code == cola._lib.COLA_ERR_INVALID
+        return
+    t2, p2, g2 = orc.nnz_split(np.asarray(ri, np.int32), np.asarray(rp, np.int32), nnz)
+    assert np.array_equal(t, t2) and np.array_equal(p, p2) and gran == g2
+
+
+@settings(**dict(SETTINGS, max_examples=200))
+@given(small_ints, small_ints, small_ints, st.booleans())
+def test_zip_create_rejects_without_crashing(ptr, idx, zid, with_val):
+    """cola_zip_create on arbitrary arrays: INVALID for malformed ones; well
+    formed ones reach the device upload (a HIP status without a GPU)."""
+    try:
+        cola.d_csr_zip(ptr, idx, zid, np.ones(len(idx), np.float32) if with_val else None)
+    except cola.ColaError as e:
+        assert e.code in (cola._lib.COLA_ERR_INVALID, cola._lib.COLA_ERR_HIP), str(e)
