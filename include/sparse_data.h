@@ -95,6 +95,9 @@ struct zip_base {
 
   zip_base(const std::vector<int> &p, const std::vector<int> &i, const std::vector<int> &z,
            const std::vector<float> &v) {
+    // cola_zip_create reads size(p)-1 ids and size(i) values.
+    if (z.size() != (p.empty() ? 0 : p.size() - 1) || (!v.empty() && v.size() != i.size()))
+      throw cola_error(COLA_ERR_INVALID, "zip: need one id per pointer interval and one value per index");
     cola_throw_if(cola_zip_create(p.data(), (int64_t)p.size(), i.data(), (int64_t)i.size(), z.data(),
                                   v.empty() ? nullptr : v.data(), &handle));
     cola_throw_if(cola_zip_device_arrays(handle, &ptr, &idx, &zip, &val, &pack, &nptr, &nnz, &n_unique));
