@@ -562,10 +562,13 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
     const int64_t P = std::max<int32_t>(1, p.nodesPerPE), U = std::max<int32_t>(1, out->U_r);
     std::vector<std::vector<WorkItem>> per(2 * K);
     int32_t pch = 0;
-    for (int64_t i = 0; i < M; ++i) {
+    // Same rows, order and hub test as the items above, so panel_hub_ptr
+    // lines up with hub_rows (the hub reduction pairs them).
+    for (int64_t oi = 0; oi < M; ++oi) {
+      const int64_t i = order[oi];
       const int64_t b = cnt[i], e = cnt[i + 1], d = e - b;
       if (d == 0) continue;
-      const bool is_hub = d > hub;
+      const bool is_hub = d > sdeg && d > hub;
       const int32_t kflag = d > kdeg ? 1 : 0;
       int64_t s = b;
       while (s < e) {

@@ -286,6 +286,27 @@ def test_hub_rows_kahan_under_schedules(tuning):
         assert rel_fro(C.cpu().numpy()[:case.M], case.ref) <= FP32_TOL, flags
 
 
+@pytest.mark.parametrize("row_order", [0, 1])
+def test_panels_with_hub_rows_below_the_short_degree(row_order):
+    """Column panels with hub_threshold < short_degree and with degree-ordered
+    items: the panel pass must chunk exactly the rows the plan's hub table
+    lists, in the same order (they share the hub reduction)."""
+    g = cola.Graph.rmat(11, 12, seed=21)
+    case = Case(g, 2, 0, 64, seed=6)
+    plan = cola.Plan(case.part, hub_threshold=8, chunk_nnz=4, short_degree=24, col_panels=3, row_order=row_order)
+    assert plan.info["num_hub_rows"] > 0
+    plan.tuning = dict(col_tile=32, rpw_short=0, rpw_long=2, blocks_per_cu=4, xcd_remap=1, split_remote=0,
+                       unroll=8, panels=1)
+    packed = torch.empty((max(plan.U_r, 1), case.dim), dtype=torch.float32, device=case.dev)
+    plan.pack(case.B, packed)
+    for flags, init in ((cola.OVERWRITE, 3.0), (cola.ACCUMULATE, 0.5)):
+        C = torch.full((case.P, case.dim), init, dtype=torch.float32, device=case.dev)
+        plan.spmm(C, case.B_local(), packed, flags=flags)
+        torch.cuda.synchronize()
+        got = C.cpu().numpy()[:case.M] - (0.0 if flags == cola.OVERWRITE else init)
+        assert rel_fro(got, case.ref) <= FP32_TOL, flags
+
+
 def test_linearity_in_B(c1):
     """A·(B1 + B2) = A·B1 + A·B2 to FP32 rounding."""
     plan = cola.Plan(c1.part)
