@@ -134,3 +134,47 @@ def test_batch_random(problems, dtype, data):
         rp, ci, va = p["g"].arrays()
         ref = orc.spmm_f64(rp, ci, va, 0, p["g"].n, dequant(p["B"], p["scales"], dtype))
         assert rel_fro(Cb, ref) <= TOL
+
+
+@settings(max_examples=60, **SETTINGS)
+@given(graphs(), st.integers(1, 6), st.data(), st.sampled_from([1, 3, 32, 40, 64, 100, 128, 256]), st.booleans())
+def test_reference_api_random(g, nGPU, data, dim, split):
+    """The reference API on random partitions: zips (with or without
+    nnz_split) through the single-stage wrapper (a plan rebuilt from the zips,
+    called twice: C accumulates) and through the literal ppl0_batch_SpMM."""
+    n = g.n
+    rank = data.draw(st.integers(0, nGPU - 1))
+    part = g.partition(nGPU, rank)
+    info = part.info()
+    P, M = info["nodesPerPE"], info["num_rows"]
+    print(f"case reference api n={n} nnz={g.nnz} nGPU={nGPU} rank={rank} dim={dim} split={split}", flush=True)
+    a = part.arrays()
+    if split:
+        tl, pl, _ = cola.nnz_split(a["csr_rowIdx_l"], a["csr_rowPtr_l"], len(a["csr_colIdx_l"]))
+        tr, pr, _ = cola.nnz_split(a["csc_colIdx_r"], a["csc_colPtr_r"], len(a["csc_rowIdx_r"]))
+    else:
+        tl, pl = a["csr_rowIdx_l"], a["csr_rowPtr_l"]
+        tr, pr = a["csc_colIdx_r"], a["csc_colPtr_r"]
+    zl = cola.d_csr_zip(pl, a["csr_colIdx_l"], tl, a["csr_val_l"])
+    zr = cola.d_csc_zip(pr, a["csc_rowIdx_r"], tr, a["csc_val_r"])
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(dim + nGPU)
+    Bh = rng.uniform(-1, 1, (P * nGPU, dim)).astype(np.float32)
+    B = torch.from_numpy(Bh).to(dev)
+    packed = torch.empty((max(zr.n_unique, 1), dim), dtype=torch.float32, device=dev)
+    rp, ci, va = g.arrays()
+    ref = orc.spmm_f64(rp, ci, va, rank * P, rank * P + M, Bh[:n])
+    C = torch.full((P, dim), 0.25, dtype=torch.float32, device=dev)
+    cola.dist_spmm_ppl_batch(C, B, packed, zl, zr, dim, P, rank)
+    cola.dist_spmm_ppl_batch(C, B, packed, zl, zr, dim, P, rank)
+    torch.cuda.synchronize()
+    got = C.cpu().numpy()
+    assert rel_fro((got[:M] - 0.25) / 2, ref) <= TOL
+    assert np.all(got[M:] == 0.25)
+    C.fill_(0.25)
+    uniq = torch.from_numpy(np.unique(np.asarray(tr, np.int32))).to(dev)
+    cola.pack_remote(B, uniq, packed, num_cols=zr.n_unique)
+    cola.ppl0_batch_SpMM(C, B[rank * P:(rank + 1) * P], packed, zl, zr, 0, 0, zl.num, zr.num, dim, P, rank)
+    torch.cuda.synchronize()
+    got = C.cpu().numpy()
+    assert rel_fro(got[:M] - 0.25, ref) <= TOL
