@@ -198,3 +198,28 @@ def test_zip_create_rejects_without_crashing(ptr, idx, zid, with_val):
         cola.d_csr_zip(ptr, idx, zid, np.ones(len(idx), np.float32) if with_val else None)
     except cola.ColaError as e:
         assert e.code in (cola._lib.COLA_ERR_INVALID, cola._lib.COLA_ERR_HIP), str(e)
+
+
+tokens = st.sampled_from(["1", "2", "3", "0", "-1", "7", "1.5", "-2e3", "nan", "inf", "x", "+4", "1e400", "", " ",
+                          "%", "%%", "99999999999", "2147483648", "\t"])
+
+
+@settings(**dict(SETTINGS, max_examples=300))
+@given(st.sampled_from(["real general", "pattern general", "integer symmetric", "real skew-symmetric",
+                        "real symmetric", "complex general", "real hermitian", "bogus general"]),
+       st.lists(tokens, min_size=0, max_size=4), st.lists(st.lists(tokens, min_size=0, max_size=4), max_size=12),
+       st.booleans())
+def test_mtx_reader_never_crashes(tmp_path_factory, header, size_line, lines, crlf):
+    """Random and corrupted MatrixMarket text: a graph or a ColaError."""
+    nl = "\r\n" if crlf else "\n"
+    text = f"%%MatrixMarket matrix coordinate {header}{nl}" + " ".join(size_line) + nl
+    text += nl.join(" ".join(t) for t in lines)
+    path = tmp_path_factory.mktemp("fz") / "f.mtx"
+    path.write_text(text)
+    try:
+        g = cola.Graph.from_mtx(str(path))
+    except cola.ColaError as e:
+        assert e.code in (cola._lib.COLA_ERR_INVALID, cola._lib.COLA_ERR_UNSUPPORTED, cola._lib.COLA_ERR_IO)
+        return
+    rp, ci, va = g.arrays()
+    assert rp[0] == 0 and np.all(np.diff(rp) >= 0) and np.all((ci >= 0) & (ci < g.n))
