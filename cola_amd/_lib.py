@@ -53,7 +53,7 @@ class PartitionArrays(ctypes.Structure):
 class PlanOpts(ctypes.Structure):
     _fields_ = [("hub_threshold", c_i32), ("chunk_nnz", c_i32), ("kahan_degree", c_i32),
                 ("max_dim", c_i32), ("short_degree", c_i32), ("col_panels", c_i32),
-                ("row_order", c_i32)]
+                ("row_order", c_i32), ("hot_cols", c_i32)]
 
 
 class PlanTuning(ctypes.Structure):
@@ -79,7 +79,8 @@ class ServerOpts(ctypes.Structure):
 class PlanInfo(ctypes.Structure):
     _fields_ = [("num_rows", c_i64), ("num_items", c_i64), ("num_hub_rows", c_i64),
                 ("num_hub_chunks", c_i64), ("nnz", c_i64), ("U_r", c_i64), ("U_l", c_i64),
-                ("device_bytes", c_i64), ("num_short_items", c_i64), ("num_long_items", c_i64)]
+                ("device_bytes", c_i64), ("num_short_items", c_i64), ("num_long_items", c_i64),
+                ("hot_nnz", c_i64), ("num_cold_items", c_i64)]
 
 
 # name -> argtypes (all return cola_status unless listed in _RESTYPES)

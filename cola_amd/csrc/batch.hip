@@ -429,6 +429,8 @@ cola_status cola_batch_create(const cola_batch_problem *probs, int n, cola_dtype
     if (dtype == COLA_FP8_E4M3 && ((p->U_l > 0 && !pr.scales) || (p->U_r > 0 && !pr.packed_scales)))
       return fail(COLA_ERR_INVALID, "batch_create: FP8 needs row scales");
     if (p->num_chunks > 0 && pr.dim > p->max_dim) return fail(COLA_ERR_INVALID, "batch_create: dim exceeds max_dim");
+    if (p->cshortb.n + p->clongb.n > 0)
+      return fail(COLA_ERR_UNSUPPORTED, "batch_create: a hot/cold plan needs two ordered passes");
     (void)eb;
   }
   auto *b = new (std::nothrow) cola_batch_s;
@@ -784,6 +786,8 @@ cola_status cola_server_add_resident(cola_server sv, cola_plan plan, const void 
   if (sv->o.dtype == COLA_FP8_E4M3 && plan->U_l > 0 && !scales)
     return fail(COLA_ERR_INVALID, "server_add_resident: FP8 needs row scales");
   if (plan->num_chunks > 0 && dim > plan->max_dim) return fail(COLA_ERR_INVALID, "server_add_resident: dim > max_dim");
+  if (plan->cshortb.n + plan->clongb.n > 0)
+    return fail(COLA_ERR_UNSUPPORTED, "server_add_resident: a hot/cold plan needs two ordered passes");
   Resident r;
   r.prob = cola_batch_problem{plan, dim, B, ld_B, scales, nullptr, dim, nullptr, nullptr, dim};
   r.out_bytes = (int64_t)plan->num_rows * dim * 4;

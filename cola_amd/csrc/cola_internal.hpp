@@ -68,6 +68,11 @@ struct HostPlan {
   // the local-then-remote schedule).
   std::vector<WorkItem> items_short_r, items_long_r;
   std::vector<int32_t> xcd_short_r, xcd_long_r;
+  // Second (cold) pass of a hot/cold plan (hot_cols > 0): items over each
+  // row's cold entries, added onto what the first pass wrote; empty otherwise.
+  std::vector<WorkItem> items_short2, items_long2;
+  std::vector<int32_t> xcd_short2, xcd_long2;
+  int64_t hot_nnz = 0;
   std::vector<int32_t> empty_rows;   // rows without entries (OVERWRITE)
   std::vector<int32_t> hub_rows;     // output row of each hub
   std::vector<int32_t> hub_chunk_ptr;// chunk range per hub (num_hubs+1)

@@ -54,6 +54,8 @@ struct cola_plan_s {
   PlanBin shortb, longb;      // every item
   PlanBin shortb_r, longb_r;  // items with remote entries (second phase)
   PlanBin emptyb;             // rows without entries as empty items (batched overwrite)
+  PlanBin cshortb, clongb;    // second (cold) pass of a hot/cold plan, else empty
+  int64_t hot_nnz = 0;
   int32_t *cols = nullptr, *empty_rows = nullptr, *hub_rows = nullptr, *hub_ptr = nullptr,
           *remote_cols = nullptr;
   float *vals = nullptr, *partials = nullptr;

@@ -432,6 +432,8 @@ static std::string part_name(const std::string &dir, const char *stem, int nGPU,
 // ---------------------------------------------------------------------------
 cola_status build_plan(const Partition &p, const cola_plan_opts &o,
                        HostPlan *out) {
+  if (o.hot_cols > 0 && o.col_panels > 1)
+    return fail(COLA_ERR_INVALID, "plan: hot_cols and col_panels are exclusive");
   const int32_t hub = o.hub_threshold > 0 ? o.hub_threshold : 1024;
   const int32_t chunk = o.chunk_nnz > 0 ? o.chunk_nnz : 512;
   const int32_t kdeg = o.kahan_degree > 0 ? o.kahan_degree : 1024;
@@ -482,6 +484,51 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
     out->U_l = (int32_t)ul;
   }
 
+  // Hot / cold split (hot_cols = H > 0): the H most referenced B rows (key c
+  // for local column c, P + u for packed row u; ties by key) are hot.  Each
+  // row's entries are reordered stably to hot-local, hot-remote, cold-local,
+  // cold-remote; hcnt / hloc count the hot and hot-local entries per row.
+  const int64_t H = o.hot_cols > 0 ? o.hot_cols : 0;
+  std::vector<int64_t> hcnt, hloc;
+  out->hot_nnz = nnz;
+  if (H > 0) {
+    const int64_t Pn = p.nodesPerPE, Un = out->U_r;
+    std::vector<int64_t> refs(Pn + Un, 0);
+    for (int32_t c : out->cols) ++refs[c >= 0 ? (int64_t)c : Pn + (int64_t)~c];
+    std::vector<int32_t> keys(Pn + Un);
+    std::iota(keys.begin(), keys.end(), 0);
+    const int64_t h = std::min<int64_t>(H, Pn + Un);
+    std::nth_element(keys.begin(), keys.begin() + (h - 1), keys.end(), [&](int32_t x, int32_t y) {
+      return refs[x] != refs[y] ? refs[x] > refs[y] : x < y;
+    });
+    std::vector<uint8_t> hot(Pn + Un, 0);
+    for (int64_t k = 0; k < h; ++k) hot[keys[k]] = 1;
+    hcnt.assign(M, 0);
+    hloc.assign(M, 0);
+#pragma omp parallel
+    {
+      std::vector<std::pair<int32_t, float>> row, cold;
+#pragma omp for schedule(dynamic, 1024)
+      for (int64_t i = 0; i < M; ++i) {
+        row.clear();
+        cold.clear();
+        for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) {
+          const int32_t c = out->cols[k];
+          const bool is_hot = hot[c >= 0 ? (int64_t)c : Pn + (int64_t)~c] != 0;
+          (is_hot ? row : cold).emplace_back(c, out->vals[k]);
+          if (is_hot && c >= 0) ++hloc[i];
+        }
+        hcnt[i] = (int64_t)row.size();
+        row.insert(row.end(), cold.begin(), cold.end());
+        for (size_t k = 0; k < row.size(); ++k) {
+          out->cols[cnt[i] + k] = row[k].first;
+          out->vals[cnt[i] + k] = row[k].second;
+        }
+      }
+    }
+    out->hot_nnz = std::accumulate(hcnt.begin(), hcnt.end(), (int64_t)0);
+  }
+
   // Work items in row order; hub rows become chunk items.  flags: bit 0 =
   // compensated sums allowed, bits 1.. = number of local entries in the item
   // (the local / remote boundary of the item's range).
@@ -490,10 +537,13 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
   out->items_long.clear();
   out->items_short_r.clear();
   out->items_long_r.clear();
+  out->items_short2.clear();
+  out->items_long2.clear();
   out->empty_rows.clear();
   out->hub_rows.clear();
   out->hub_chunk_ptr.assign(1, 0);
   int32_t nchunks = 0;
+  std::vector<WorkItem> unused;
   auto add = [&](std::vector<WorkItem> &all, std::vector<WorkItem> &rem, int32_t outr, int64_t b,
                  int64_t e, int64_t mid, int32_t kflag) {
     const int64_t nloc = std::max<int64_t>(0, std::min(e, mid) - b);
@@ -516,6 +566,32 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
       continue;
     }
     const int32_t kflag = d > kdeg ? 1 : 0;
+    if (H > 0) {
+      // Hot part [b, hb) in the first pass, cold part [hb, e) in the second;
+      // each part keeps its local entries first (hmid / cmid bound them).
+      // The first pass also gets its remote-entry items, so the
+      // local-then-remote phases can run it; the second pass runs whole.
+      const int64_t hb = b + hcnt[i], hmid = b + hloc[i], cmid = hb + (lcnt[i] - hloc[i]);
+      if (d > sdeg && d > hub) {
+        for (int64_t s = b; s < hb; s += chunk) {
+          add(out->items_long, out->items_long_r, -(nchunks + 1), s, std::min<int64_t>(hb, s + chunk), hmid, kflag);
+          ++nchunks;
+        }
+        for (int64_t s = hb; s < e; s += chunk) {
+          add(out->items_long2, unused, -(nchunks + 1), s, std::min<int64_t>(e, s + chunk), cmid, kflag);
+          ++nchunks;
+        }
+        out->hub_rows.push_back((int32_t)i);
+        out->hub_chunk_ptr.push_back(nchunks);
+      } else {
+        // Every row gets a first-pass item (possibly empty: it writes the
+        // row so the second pass can add onto it).
+        if (hb - b <= sdeg) add(out->items_short, out->items_short_r, (int32_t)i, b, hb, hmid, kflag);
+        else add(out->items_long, out->items_long_r, (int32_t)i, b, hb, hmid, kflag);
+        if (e > hb) add(e - hb <= sdeg ? out->items_short2 : out->items_long2, unused, (int32_t)i, hb, e, cmid, kflag);
+      }
+      continue;
+    }
     if (d <= sdeg) {
       add(out->items_short, out->items_short_r, (int32_t)i, b, e, mid, kflag);
     } else if (d <= hub) {
@@ -549,6 +625,8 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
   split(out->items_long, &out->xcd_long);
   split(out->items_short_r, &out->xcd_short_r);
   split(out->items_long_r, &out->xcd_long_r);
+  split(out->items_short2, &out->xcd_short2);
+  split(out->items_long2, &out->xcd_long2);
 
   // Column panels.
   const int32_t K = o.col_panels > 1 ? o.col_panels : 0;

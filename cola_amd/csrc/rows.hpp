@@ -138,7 +138,8 @@ struct RowArgs {
   int rpw;        // teams (items in flight) per wave, power of two <= 64/G
   int xcd_remap;  // 1: block b walks XCD (b % 8)'s item range
   int phase;      // 0: all entries; 1: local entries only; 2: remote entries
-                  // only, added to what phase 1 wrote
+                  // only, added to what phase 1 wrote; 3: all entries, rows
+                  // added to what the first pass of a hot/cold plan wrote
 };
 
 // One step of a wave: the `rpw` items it0 .. it0+rpw-1 (those below ie), one
@@ -286,7 +287,7 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
     bool add_old;
     if (item.x >= 0) {
       dst = a.C + (int64_t)item.x * a.ldC;
-      add_old = a.accumulate != 0 || a.phase == 2;
+      add_old = a.accumulate != 0 || a.phase >= 2;
     } else {
       dst = a.partials + (int64_t)(-item.x - 1) * a.pstride;
       add_old = a.phase == 2;

@@ -386,7 +386,7 @@ static void free_zip(cola_zip_s *z) {
 
 static void free_plan(cola_plan_s *p) {
   if (!p) return;
-  for (PlanBin *b : {&p->shortb, &p->longb, &p->shortb_r, &p->longb_r, &p->emptyb}) {
+  for (PlanBin *b : {&p->shortb, &p->longb, &p->shortb_r, &p->longb_r, &p->emptyb, &p->cshortb, &p->clongb}) {
     (void)hipFree(b->items);
     (void)hipFree(b->xcd);
   }
@@ -543,11 +543,15 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
     const int S = 64 / sh.g;
     // phase 0: every entry at once; with split_remote, phase 1 (local
     // entries over this partition's B slice) then phase 2 (remote entries
-    // over the packed buffer, added on top).
+    // over the packed buffer, added on top).  A hot/cold plan runs those over
+    // its hot items, then phase 3 over the cold items (added on top), all
+    // inside this column tile so the hot rows stay in L2.
+    const bool cold = p->cshortb.n + p->clongb.n > 0;
     const bool split = t.split_remote && p->U_r > 0;
-    for (int phase : {split ? 1 : 0, split ? 2 : -1}) {
+    for (int phase : {split ? 1 : 0, split ? 2 : -1, cold ? 3 : -1}) {
       if (phase < 0) continue;
-      const PlanBin *bins[2] = {phase == 2 ? &p->longb_r : &p->longb, phase == 2 ? &p->shortb_r : &p->shortb};
+      const PlanBin *bins[2] = {phase == 2 ? &p->longb_r : (phase == 3 ? &p->clongb : &p->longb),
+                                phase == 2 ? &p->shortb_r : (phase == 3 ? &p->cshortb : &p->shortb)};
       for (int bi = 0; bi < 2; ++bi) {
         const PlanBin *bin = bins[bi];
         if (bin->n == 0) continue;
@@ -785,7 +789,7 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
   cola_plan_opts o{};
   if (opts) o = *opts;
   if (o.hub_threshold < 0 || o.chunk_nnz < 0 || o.kahan_degree < 0 || o.max_dim < 0 || o.short_degree < 0 ||
-      o.col_panels < 0 || o.col_panels > 64 || o.row_order < 0 || o.row_order > 1)
+      o.col_panels < 0 || o.col_panels > 64 || o.row_order < 0 || o.row_order > 1 || o.hot_cols < 0)
     return fail(COLA_ERR_INVALID, "plan_create: negative option");
   return plan_from_partition(part->p, o, out);
 }
@@ -819,10 +823,16 @@ static cola_status plan_from_partition(const cola::Partition &part, const cola_p
   p->longb.n = (int64_t)il.size();
   p->shortb_r.n = (int64_t)isr.size();
   p->longb_r.n = (int64_t)ilr.size();
+  std::vector<int4> is2 = to_int4(hp.items_short2), il2 = to_int4(hp.items_long2);
+  p->cshortb.n = (int64_t)is2.size();
+  p->clongb.n = (int64_t)il2.size();
+  p->hot_nnz = hp.hot_nnz;
   if ((st = upload(is, &p->shortb.items)) || (st = upload(hp.xcd_short, &p->shortb.xcd)) ||
       (st = upload(il, &p->longb.items)) || (st = upload(hp.xcd_long, &p->longb.xcd)) ||
       (st = upload(isr, &p->shortb_r.items)) || (st = upload(hp.xcd_short_r, &p->shortb_r.xcd)) ||
       (st = upload(ilr, &p->longb_r.items)) || (st = upload(hp.xcd_long_r, &p->longb_r.xcd)) ||
+      (st = upload(is2, &p->cshortb.items)) || (st = upload(hp.xcd_short2, &p->cshortb.xcd)) ||
+      (st = upload(il2, &p->clongb.items)) || (st = upload(hp.xcd_long2, &p->clongb.xcd)) ||
       (st = upload(hp.cols, &p->cols)) || (st = upload(hp.vals, &p->vals)) ||
       (st = upload(hp.empty_rows, &p->empty_rows)) || (st = upload(hp.hub_rows, &p->hub_rows)) ||
       (st = upload(hp.hub_chunk_ptr, &p->hub_ptr)) || (st = upload(hp.remote_cols, &p->remote_cols))) {
@@ -867,7 +877,8 @@ static cola_status plan_from_partition(const cola::Partition &part, const cola_p
       return st;
     }
   }
-  p->device_bytes += (int64_t)((is.size() + il.size() + isr.size() + ilr.size()) * sizeof(int4) + 36 * 4 + hp.cols.size() * 4 +
+  p->device_bytes += (int64_t)((is.size() + il.size() + isr.size() + ilr.size() + is2.size() + il2.size()) *
+                                  sizeof(int4) + 54 * 4 + hp.cols.size() * 4 +
                               hp.vals.size() * 4 + hp.empty_rows.size() * 4 + hp.hub_rows.size() * 4 +
                               hp.hub_chunk_ptr.size() * 4 + hp.remote_cols.size() * 4 +
                               (size_t)chunks_alloc * p->pstride * 4);
@@ -950,7 +961,7 @@ extern "C" {
 cola_status cola_plan_get_info(cola_plan p, cola_plan_info *info) {
   if (!p || !info) return fail(COLA_ERR_INVALID, "plan_info: NULL argument");
   info->num_rows = p->num_rows;
-  info->num_items = p->shortb.n + p->longb.n;
+  info->num_items = p->shortb.n + p->longb.n + p->cshortb.n + p->clongb.n;
   info->num_hub_rows = p->num_hubs;
   info->num_hub_chunks = p->num_chunks;
   info->nnz = p->nnz;
@@ -959,6 +970,8 @@ cola_status cola_plan_get_info(cola_plan p, cola_plan_info *info) {
   info->device_bytes = p->device_bytes;
   info->num_short_items = p->shortb.n;
   info->num_long_items = p->longb.n;
+  info->hot_nnz = p->hot_nnz;
+  info->num_cold_items = p->cshortb.n + p->clongb.n;
   return COLA_OK;
 }
 

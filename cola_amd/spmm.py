@@ -148,9 +148,9 @@ class Plan:
     """Degree-binned, atomic-free plan of one partition (device resident)."""
 
     def __init__(self, partition, hub_threshold=0, chunk_nnz=0, kahan_degree=0, max_dim=0, short_degree=0,
-                 col_panels=0, row_order=0):
+                 col_panels=0, row_order=0, hot_cols=0):
         o = _lib.PlanOpts(int(hub_threshold), int(chunk_nnz), int(kahan_degree), int(max_dim), int(short_degree),
-                          int(col_panels), int(row_order))
+                          int(col_panels), int(row_order), int(hot_cols))
         h = ctypes.c_void_p()
         check(lib().cola_plan_create(partition.handle, ctypes.byref(o), ctypes.byref(h)))
         self._h = h

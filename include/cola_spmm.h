@@ -294,6 +294,12 @@ typedef struct {
                             one pass per range) for cache reuse; 0 = none   */
   int32_t row_order;     /* order of work items: 0 = row order, 1 = rows by
                             decreasing entry count                          */
+  int32_t hot_cols;      /* H > 0: the H most referenced B rows (local or
+                            packed) are "hot"; each row's hot entries run in
+                            a first pass that writes C and its cold entries
+                            in a second pass that adds onto it, so the hot
+                            rows stay in L2 during the first pass.  Not with
+                            col_panels or the batched form; 0 = one pass   */
 } cola_plan_opts;
 cola_status cola_plan_create(cola_partition p, const cola_plan_opts *opts,
                              cola_plan *out);
@@ -301,6 +307,8 @@ typedef struct {
   int64_t num_rows, num_items, num_hub_rows, num_hub_chunks, nnz, U_r, U_l;
   int64_t device_bytes;
   int64_t num_short_items, num_long_items;
+  int64_t hot_nnz;       /* entries of the first (hot) pass; nnz if one pass */
+  int64_t num_cold_items;/* items of the second pass                         */
 } cola_plan_info;
 cola_status cola_plan_get_info(cola_plan plan, cola_plan_info *info);
 

@@ -307,6 +307,42 @@ def test_panels_with_hub_rows_below_the_short_degree(row_order):
         assert rel_fro(got, case.ref) <= FP32_TOL, flags
 
 
+@pytest.mark.parametrize("hot_cols", [1, 300, 1 << 30])
+@pytest.mark.parametrize("tuning", [dict(col_tile=0, rpw_long=1, unroll=8), dict(col_tile=32, rpw_long=4, unroll=16),
+                                    dict(col_tile=16, rpw_short=8, rpw_long=8, unroll=16, split_remote=1)])
+def test_hot_cold_passes(c1, hot_cols, tuning):
+    """Two-pass plans (hot entries write C, cold entries add onto it; hub
+    chunks of both passes reduced together) give the one-pass result under
+    every flag; split_remote is ignored for them."""
+    plan = cola.Plan(c1.part, hot_cols=hot_cols, hub_threshold=200, chunk_nnz=64, kahan_degree=100, row_order=1)
+    info = plan.info
+    assert 0 < info["hot_nnz"] <= info["nnz"]
+    assert (info["num_cold_items"] > 0) == (info["hot_nnz"] < info["nnz"])
+    full = dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=4, xcd_remap=0, split_remote=0, unroll=8, panels=0)
+    full.update(tuning)
+    plan.tuning = full
+    packed = torch.empty((plan.U_r, c1.dim), dtype=torch.float32, device=c1.dev)
+    plan.pack(c1.B, packed)
+    for flags, init in ((cola.OVERWRITE, 7.0), (cola.ACCUMULATE, 0.5), (cola.OVERWRITE | cola.KAHAN, 7.0)):
+        C = torch.full((c1.P, c1.dim), init, dtype=torch.float32, device=c1.dev)
+        plan.spmm(C, c1.B_local(), packed, flags=flags)
+        torch.cuda.synchronize()
+        got = C.cpu().numpy()
+        assert rel_fro(got[:c1.M] - (init if flags == cola.ACCUMULATE else 0.0), c1.ref) <= FP32_TOL, flags
+        assert np.all(got[c1.M:] == init)
+
+
+def test_hot_cold_plans_are_not_batched(c1):
+    plan = cola.Plan(c1.part, hot_cols=100)
+    C = torch.zeros((c1.P, c1.dim), dtype=torch.float32, device=c1.dev)
+    packed = torch.empty((plan.U_r, c1.dim), dtype=torch.float32, device=c1.dev)
+    with pytest.raises(cola.ColaError) as e:
+        cola.Batch([dict(plan=plan, B=c1.B_local(), packed=packed, C=C, dim=c1.dim)], dtype=cola.F32)
+    assert e.value.code == cola._lib.COLA_ERR_UNSUPPORTED
+    with pytest.raises(cola.ColaError):
+        cola.Plan(c1.part, hot_cols=100, col_panels=2)
+
+
 def test_linearity_in_B(c1):
     """A·(B1 + B2) = A·B1 + A·B2 to FP32 rounding."""
     plan = cola.Plan(c1.part)

@@ -57,7 +57,8 @@ def graphs(draw, max_n=2500, max_deg=40):
 plan_opts = st.fixed_dictionaries(dict(
     hub_threshold=st.sampled_from([0, 8, 64]), chunk_nnz=st.sampled_from([0, 4, 32]),
     kahan_degree=st.sampled_from([0, 4]), short_degree=st.sampled_from([0, 3, 16]),
-    col_panels=st.sampled_from([0, 0, 3]), row_order=st.sampled_from([0, 1])))
+    col_panels=st.sampled_from([0, 0, 3]), row_order=st.sampled_from([0, 1]),
+    hot_cols=st.sampled_from([0, 0, 1, 50, 1 << 30])))
 tunings = st.fixed_dictionaries(dict(
     col_tile=st.sampled_from([0, 8, 16, 32, 64]), rpw_short=st.sampled_from([0, 1, 2, 4, 8, 16, 32, 64]),
     rpw_long=st.sampled_from([0, 1, 2, 4, 8]), blocks_per_cu=st.sampled_from([1, 2, 4, 8]),
@@ -77,6 +78,8 @@ def test_plan_path_random(g, nGPU, data, dim, dtype, opts, tuning, flags):
     P, M = info["nodesPerPE"], info["num_rows"]
     print(f"case n={n} nnz={g.nnz} nGPU={nGPU} rank={rank} dim={dim} dtype={dtype} opts={opts} "
           f"tuning={tuning} flags={flags}", flush=True)
+    if opts["col_panels"] > 1:
+        opts = dict(opts, hot_cols=0)  # the two schedules are exclusive
     plan = cola.Plan(part, max_dim=max(dim, 256), **opts)
     plan.tuning = tuning
     dev = torch.device("cuda:0")
