@@ -14,12 +14,17 @@
 //   quantize_rows_kernel  FP32 rows -> BF16 / FP16 / FP8-e4m3 (+ row scale)
 //   error_norms_kernel    FP64 error norms of C against an FP64 reference
 //
+//   zero_matrix_kernel    zeroes C before the column-panel passes
+//   fill_f64_kernel       device fill of the FP64 norm accumulators
+//
+// The plan also caches itself behind the reference's single-stage wrapper
+// (cola_dist_spmm_ppl_batch): a plan rebuilt from the two zips on first use.
+//
 // SpMM is HBM/cache-bandwidth bound (about 2 FLOP per 8+ bytes), so nothing
 // here uses MFMA.  The dense dimension is spread across the lanes of a row
-// group (G lanes, 4 consecutive elements per lane per vector), several row
-// groups share a wave, and (column, value) pairs are loaded once per 64
-// entries with one coalesced load and broadcast to the groups with
-// ds_bpermute.
+// group (G lanes, 16 bytes per lane per vector), several row groups share a
+// wave, and (column, value) pairs are staged per team lane with coalesced
+// loads and broadcast to the groups with ds_bpermute (rows.hpp).
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
