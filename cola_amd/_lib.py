@@ -108,6 +108,7 @@ SIGNATURES = {
     "cola_zip_device_arrays": [c_vp, ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), ctypes.POINTER(c_vp),
                                ctypes.POINTER(c_vp), ctypes.POINTER(c_vp), P_i64, P_i64, P_i64],
     "cola_zip_destroy": [c_vp],
+    "cola_zip_invalidate": [c_vp],
     "cola_pack_remote": [c_vp, c_i64, c_vp, c_vp, c_i64, c_int, c_int, c_vp, c_i64, c_vp, c_vp],
     "cola_ppl0_batch_spmm": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                              c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_vp],

@@ -32,10 +32,11 @@ struct cola_zip_s {
           *uniq = nullptr;
   float *val = nullptr;
   int64_t nptr = 0, nnz = 0, n_unique = 0;
-  // Plan built from this (local) zip and fast_peer (the remote zip) by the
-  // single-stage wrapper, reused while its arguments match.
+  uint64_t serial = 0;  // unique per created zip (never reused)
+  // Plan built from this (local) zip and the remote zip with serial
+  // fast_peer by the single-stage wrapper, reused while its arguments match.
   cola_plan_s *fast_plan = nullptr;
-  const cola_zip_s *fast_peer = nullptr;
+  uint64_t fast_peer = 0;
   int fast_P = -1, fast_peid = -1;
 };
 

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -603,6 +604,8 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
 }  // namespace
 
 static cola_status plan_from_partition(const cola::Partition &part, const cola_plan_opts &o, cola_plan_s **out);
+// Guards the plans the single-stage wrapper caches in zips.
+static std::mutex g_zip_plan_mu;
 static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int peid, int dim, cola_plan_s **out);
 
 // ---------------------------------------------------------------------------
@@ -626,6 +629,8 @@ cola_status cola_zip_create(const int32_t *ptr, int64_t nptr, const int32_t *idx
   }
   auto *z = new (std::nothrow) cola_zip_s;
   if (!z) return fail(COLA_ERR_NOMEM, "zip_create: out of host memory");
+  static std::atomic<uint64_t> next_serial{1};
+  z->serial = next_serial.fetch_add(1);
   z->nptr = nptr;
   z->nnz = nnz;
   std::vector<int32_t> vptr(ptr, ptr + nptr), vidx(idx, idx + nnz), vzip(zip, zip + nz);
@@ -788,6 +793,15 @@ cola_status cola_dist_spmm_ppl_batch(float *output, const float *B_global, float
                              COLA_ACCUMULATE, stream);
 }
 
+cola_status cola_zip_invalidate(cola_zip z) {
+  if (!z) return fail(COLA_ERR_INVALID, "zip_invalidate: NULL handle");
+  std::lock_guard<std::mutex> lock(g_zip_plan_mu);
+  free_plan(z->fast_plan);
+  z->fast_plan = nullptr;
+  z->fast_peer = 0;
+  return COLA_OK;
+}
+
 cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, cola_plan *out) {
   if (!part || !out) return fail(COLA_ERR_INVALID, "plan_create: NULL argument");
   *out = nullptr;
@@ -907,9 +921,9 @@ static cola_status download(const T *d, int64_t n, std::vector<T> *h) {
 // for dims of 64 and more, 4 rows per wave on long rows, 2 blocks per CU, 16
 // loads in flight (the tuned schedule of the 2^22-node partition).
 static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int peid, int dim, cola_plan_s **out) {
-  static std::mutex mu;
-  std::lock_guard<std::mutex> lock(mu);
-  if (l->fast_plan && l->fast_peer == r && l->fast_P == P && l->fast_peid == peid && dim <= l->fast_plan->max_dim) {
+  std::lock_guard<std::mutex> lock(g_zip_plan_mu);
+  if (l->fast_plan && l->fast_peer == r->serial && l->fast_P == P && l->fast_peid == peid &&
+      dim <= l->fast_plan->max_dim) {
     *out = l->fast_plan;
     return COLA_OK;
   }
@@ -954,7 +968,7 @@ static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int 
   if ((st = plan_from_partition(part, o, &p))) return st;
   p->tune = cola_plan_tuning{dim >= 64 ? 32 : 0, 0, 4, 2, 0, 0, 16, 0};
   l->fast_plan = p;
-  l->fast_peer = r;
+  l->fast_peer = r->serial;
   l->fast_P = P;
   l->fast_peid = peid;
   *out = p;

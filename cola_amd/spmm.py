@@ -77,6 +77,11 @@ class _Zip:
     def handle(self):
         return self._h
 
+    def invalidate(self):
+        """Drops the plan the single-stage wrapper cached from this (local)
+        zip, e.g. after writing new values into its device arrays."""
+        check(lib().cola_zip_invalidate(self._h))
+
     @property
     def num(self):
         """Number of (virtual) rows / columns."""

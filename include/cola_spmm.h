@@ -263,13 +263,19 @@ cola_status cola_ppl0_batch_spmm(
  * as ppl0_batch_SpMM over the virtual rows / columns, computed through a
  * degree-binned plan rebuilt from the two zips on the first call and cached
  * in csr_zip_l (no atomics, deterministic order); cola_ppl0_batch_spmm is the
- * literal virtual-row kernel.  Calls on one zip from several threads are
- * serialised while the plan is built; concurrent launches race on its hub
- * workspace. */
+ * literal virtual-row kernel.  The plan copies the zips' arrays at that
+ * first call: after writing new values or indices into a zip's device arrays,
+ * call cola_zip_invalidate on the local zip (or create new zips).  Calls on
+ * one zip from several threads are serialised while the plan is built;
+ * concurrent launches race on its hub workspace. */
 cola_status cola_dist_spmm_ppl_batch(float *output, const float *B_global,
                                      float *packed, cola_zip csr_zip_l,
                                      cola_zip csc_zip_r, int dim,
                                      int nodesPerPE, int peid, void *stream);
+/* Drops the plan cola_dist_spmm_ppl_batch cached in a (local) zip; the next
+ * call rebuilds it from the zips' current arrays.  No counterpart in the
+ * reference (whose wrapper reads the zips on every call). */
+cola_status cola_zip_invalidate(cola_zip csr_zip_l);
 
 /* Degree-binned, atomic-free partition plan (replaces nnz_split's virtual
  * rows for the fast path): for every owned row, local nnz and remote nnz are

@@ -123,6 +123,35 @@ def test_reference_path_matches_fp32_restatement(c1, literal):
     assert rel_fro(C, Cr.astype(np.float64)) <= FP32_TOL
 
 
+def test_reference_wrapper_follows_new_values_after_invalidate(c1):
+    """Values written into the zips' device arrays reach the result once the
+    cached plan is invalidated (2x the values: 2x the product)."""
+    a = c1.part.arrays()
+    tl, pl, _ = cola.nnz_split(a["csr_rowIdx_l"], a["csr_rowPtr_l"], len(a["csr_colIdx_l"]))
+    tr, pr, _ = cola.nnz_split(a["csc_colIdx_r"], a["csc_colPtr_r"], len(a["csc_rowIdx_r"]))
+    zl = cola.d_csr_zip(pl, a["csr_colIdx_l"], tl, a["csr_val_l"])
+    zr = cola.d_csc_zip(pr, a["csc_rowIdx_r"], tr, a["csc_val_r"])
+    packed = torch.empty((max(zr.n_unique, 1), c1.dim), dtype=torch.float32, device=c1.dev)
+    C = torch.zeros((c1.P, c1.dim), dtype=torch.float32, device=c1.dev)
+    cola.dist_spmm_ppl_batch(C, c1.B, packed, zl, zr, c1.dim, c1.P, c1.rank)
+    torch.cuda.synchronize()
+    assert rel_fro(C.cpu().numpy()[:c1.M], c1.ref) <= FP32_TOL
+    # double every value in place through the exposed device pointers (a
+    # device-to-device hipMemcpy; torch and the library share the runtime)
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    for ptr, vals in ((zl.d_csr_val, a["csr_val_l"]), (zr.d_csc_val, a["csc_val_r"])):
+        src = torch.from_numpy(np.asarray(vals, np.float32) * 2).to(c1.dev)
+        torch.cuda.synchronize()
+        assert hip.hipMemcpy(ctypes.c_void_p(ptr), ctypes.c_void_p(src.data_ptr()),
+                             ctypes.c_size_t(src.numel() * 4), 3) == 0
+    zl.invalidate()
+    C.zero_()
+    cola.dist_spmm_ppl_batch(C, c1.B, packed, zl, zr, c1.dim, c1.P, c1.rank)
+    torch.cuda.synchronize()
+    assert rel_fro(C.cpu().numpy()[:c1.M] / 2, c1.ref) <= FP32_TOL
+
+
 def test_reference_wrapper_accumulates_and_reuses_its_plan(c1):
     """The wrapper adds into C on every call (cuh:141,196), through the plan
     cached from the zips after the first call."""
