@@ -475,8 +475,9 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
                       aligned(c.C, 16) && aligned(c.Bl, 16) && (p->U_r == 0 || aligned(c.Bp, 16));
   // Column tile: a multiple of the vector width (so tiles stay aligned).
   int tile = c.dim;
-  if (t.col_tile > 0 && t.col_tile < c.dim && vec_ok && c.dim % ev == 0) {
-    tile = std::max(ev, t.col_tile / ev * ev);
+  const int want = t.col_tile < 0 ? 128 / dtype_bytes(c.dtype) : t.col_tile;
+  if (want > 0 && want < c.dim && vec_ok && c.dim % ev == 0) {
+    tile = std::max(ev, want / ev * ev);
   }
   const int accumulate = (c.flags & COLA_OVERWRITE) ? 0 : 1;
   const int kahan = (c.flags & COLA_KAHAN) ? 1 : 0;
@@ -828,6 +829,11 @@ static cola_status plan_from_partition(const cola::Partition &part, const cola_p
   p->max_dim = hp.max_dim;
   p->pstride = (hp.max_dim + 15) / 16 * 16;
   p->nnz = hp.nnz;
+  // Default schedule by size: large partitions gather from far more B rows
+  // than the L2s hold and want 128-byte column tiles at low occupancy with 16
+  // loads in flight; small ones want every column in one pass and many waves.
+  p->tune = hp.nnz >= (int64_t(1) << 22) ? cola_plan_tuning{-1, 0, 4, 2, 0, 0, 16, 0}
+                                         : cola_plan_tuning{0, 0, 1, 8, 1, 0, 8, 0};
   p->num_empty = (int64_t)hp.empty_rows.size();
   p->num_hubs = (int64_t)hp.hub_rows.size();
   p->num_chunks = hp.hub_chunk_ptr.back();
@@ -917,9 +923,7 @@ static cola_status download(const T *d, int64_t n, std::vector<T> *h) {
 // workspace stay the same.  The partition is recovered from the zips: the
 // virtual rows (columns) that nnz_split cut from one row (column) are
 // consecutive with the same id, so they are merged back; entries keep their
-// order.  Items in decreasing-degree order; schedule: 128-byte column tiles
-// for dims of 64 and more, 4 rows per wave on long rows, 2 blocks per CU, 16
-// loads in flight (the tuned schedule of the 2^22-node partition).
+// order.  Items in decreasing-degree order; the plan's default schedule.
 static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int peid, int dim, cola_plan_s **out) {
   std::lock_guard<std::mutex> lock(g_zip_plan_mu);
   if (l->fast_plan && l->fast_peer == r->serial && l->fast_P == P && l->fast_peid == peid &&
@@ -966,7 +970,6 @@ static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int 
   o.row_order = 1;
   cola_plan_s *p = nullptr;
   if ((st = plan_from_partition(part, o, &p))) return st;
-  p->tune = cola_plan_tuning{dim >= 64 ? 32 : 0, 0, 4, 2, 0, 0, 16, 0};
   l->fast_plan = p;
   l->fast_peer = r->serial;
   l->fast_P = P;
@@ -996,7 +999,7 @@ cola_status cola_plan_get_info(cola_plan p, cola_plan_info *info) {
 
 cola_status cola_plan_set_tuning(cola_plan p, const cola_plan_tuning *t) {
   if (!p || !t) return fail(COLA_ERR_INVALID, "plan_set_tuning: NULL argument");
-  if (t->col_tile < 0 || t->rpw_short < 0 || t->rpw_short > 64 || t->rpw_long < 0 || t->rpw_long > 64 ||
+  if (t->col_tile < -1 || t->rpw_short < 0 || t->rpw_short > 64 || t->rpw_long < 0 || t->rpw_long > 64 ||
       t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || (t->xcd_remap != 0 && t->xcd_remap != 1) ||
       (t->split_remote != 0 && t->split_remote != 1) || (t->unroll != 0 && t->unroll != 8 && t->unroll != 16) ||
       (t->panels != 0 && t->panels != 1))

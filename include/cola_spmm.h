@@ -318,21 +318,27 @@ typedef struct {
 } cola_plan_info;
 cola_status cola_plan_get_info(cola_plan plan, cola_plan_info *info);
 
-/* Launch parameters of the plan's kernels (the autotuner's search space). */
+/* Launch parameters of the plan's kernels (the autotuner's search space).
+ * A new plan with at least 2^22 entries starts with {-1, 0, 4, 2, 0, 0, 16,
+ * 0}: 128-byte column tiles, 4 rows per wave on long rows, 2 blocks per CU,
+ * 16 loads in flight (the tuned schedule of the 2^22-node partition); a
+ * smaller one with {0, 0, 1, 8, 1, 0, 8, 0}: all columns in one pass, one
+ * row per wave on long rows, 8 blocks per CU. */
 typedef struct {
-  int32_t col_tile;      /* dense columns per pass over B (0 = all of dim);
-                            rounded down to the 16-byte vector width        */
+  int32_t col_tile;      /* dense columns per pass over B (0 = all of dim;
+                            -1 = 128 bytes of the storage dtype); rounded
+                            down to the 16-byte vector width                */
   int32_t rpw_short;     /* short-row bin: rows in flight per wave (power of
                             two; 0 = one row per lane group)                */
   int32_t rpw_long;      /* long-row bin: rows in flight per wave (0 = 1)   */
   int32_t blocks_per_cu; /* grid size in 256-thread blocks per CU (0 = 8)   */
-  int32_t xcd_remap;     /* 1: each XCD walks a contiguous row range
-                            (default); 0: plain block order                 */
+  int32_t xcd_remap;     /* 1: each XCD walks a contiguous row range; 0:
+                            plain block order                               */
   int32_t split_remote;  /* 1: local entries first (B_p only), then remote
                             entries (packed only) added on top; 0: both in
                             one pass (default)                              */
   int32_t unroll;        /* row-gather loads in flight per lane group: 8
-                            (default, also 0) or 16                         */
+                            (also 0) or 16                                  */
   int32_t panels;        /* 1: run the column-panel schedule (plans built
                             with col_panels > 1): C is zeroed (overwrite),
                             then one accumulating pass per panel; 0: off    */
