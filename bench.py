@@ -261,6 +261,8 @@ def main():
     ap.add_argument("--nparts", type=int, default=8)
     ap.add_argument("--rank", type=int, default=0)
     ap.add_argument("--dim", type=int, default=128)
+    ap.add_argument("--dtype", choices=["f32", "bf16", "f16", "fp8_e4m3"], default="f32",
+                    help="storage dtype of B in the main step (the metric is quoted for f32)")
     ap.add_argument("--row-order", type=int, default=1, help="plan item order: 0 row order, 1 by degree")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -313,9 +315,23 @@ def main():
 
     gen = torch.Generator(device=dev).manual_seed(1234)
     B = torch.rand((P * args.nparts, N), generator=gen, device=dev, dtype=torch.float32).mul_(2).sub_(1)
-    B_local = B[lb:lb + P]
-    packed = torch.empty((max(U_r, 1), N), dtype=torch.float32, device=dev)
+    # Storage dtype of B for the main step (FP32 unless --dtype says otherwise).
+    dt = {"f32": cola.F32, "bf16": cola.BF16, "f16": cola.F16, "fp8_e4m3": cola.FP8_E4M3}[args.dtype]
+    s_B = cola.DTYPE_BYTES[dt]
+    s_sc = 4 if dt == cola.FP8_E4M3 else 0
+    Bs, Bsc = (B, None) if dt == cola.F32 else cola.quantize_rows(B, dt)
+    B_local = Bs[lb:lb + P]
+    sc_local = None if Bsc is None else Bsc[lb:lb + P]
+    tdt = {cola.F32: torch.float32, cola.BF16: torch.uint16, cola.F16: torch.uint16, cola.FP8_E4M3: torch.uint8}[dt]
+    packed = torch.empty((max(U_r, 1), N), dtype=tdt, device=dev)
+    psc = torch.empty(max(U_r, 1), dtype=torch.float32, device=dev) if dt == cola.FP8_E4M3 else None
     C = torch.empty((P, N), dtype=torch.float32, device=dev)
+
+    def pack():
+        plan.pack(Bs, packed, dtype=dt, scales_global=Bsc, packed_scales=psc)
+
+    def spmm():
+        plan.spmm(C, B_local, packed, dtype=dt, flags=cola.OVERWRITE, scales_local=sc_local, packed_scales=psc)
 
     if args.copy_calibrate_mib > 0:
         nel = args.copy_calibrate_mib * (1 << 20) // 4
@@ -326,7 +342,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    plan.pack(B, packed)
+    pack()
     # The committed counter profile (HBM bytes per launch) holds only for the
     # workload and schedule it was measured with.
     tfile = os.path.join(ROOT, "profiles", "plan_rows_f32_rmat22_traffic.json")
@@ -335,14 +351,15 @@ def main():
         with open(tfile) as f:
             prof = json.load(f)
         if not (prof.get("row_order", 0) == args.row_order and N == 128 and args.log_n == 22
-                and args.degree == 32 and args.nparts == 8 and args.rank == 0):
+                and args.degree == 32 and args.nparts == 8 and args.rank == 0 and dt == cola.F32):
             prof = None
     tuning_source = args.tuning
     if args.tuning == "profile" and prof is None:
         tuning_source = "auto"
     if tuning_source == "auto":
-        tuning, tune_ms = plan.autotune(C, B_local, packed, iters=3)
-        log(f"autotuned f32: {tuning} ({tune_ms:.3f} ms)")
+        tuning, tune_ms = plan.autotune(C, B_local, packed, dtype=dt, scales_local=sc_local, packed_scales=psc,
+                                        iters=3)
+        log(f"autotuned {args.dtype}: {tuning} ({tune_ms:.3f} ms)")
     else:
         if tuning_source == "profile":
             tj = prof["tuning"]
@@ -354,14 +371,14 @@ def main():
             tj = json.loads(args.tuning)
         plan.tuning = tj
         tuning = plan.tuning
-        log(f"fixed f32 tuning ({tuning_source}): {tuning}")
+        log(f"fixed {args.dtype} tuning ({tuning_source}): {tuning}")
     if args.save_tuning:
         with open(args.save_tuning, "w") as f:
             json.dump(tuning, f)
 
     for _ in range(args.warmup):
-        plan.pack(B, packed)
-        plan.spmm(C, B_local, packed, flags=cola.OVERWRITE)
+        pack()
+        spmm()
     torch.cuda.synchronize()
     ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
     start = torch.cuda.Event(enable_timing=True)
@@ -370,23 +387,23 @@ def main():
     start.record()
     for k in range(args.steps):
         ev[k][0].record()
-        plan.pack(B, packed)
+        pack()
         ev[k][1].record()
-        plan.spmm(C, B_local, packed, flags=cola.OVERWRITE)
+        spmm()
         ev[k][2].record()
     end.record()
     torch.cuda.synchronize()
     ms = start.elapsed_time(end) / args.steps
     pack_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
     spmm_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
-    balg = bytes_alg(M, nnz_p, U_l + U_r, N, 4)
+    balg = bytes_alg(M, nnz_p, U_l + U_r, N, s_B, s_sc)
     value = balg / (ms * 1e-3) / 1e9
     # Row kernel's own algorithmic bytes: entry lists + work items + distinct
     # local B rows + packed rows + C.
-    kbytes = nnz_p * 8 + plan.info["num_items"] * 16 + (U_l + U_r) * N * 4 + M * N * 4
+    kbytes = nnz_p * 8 + plan.info["num_items"] * 16 + (U_l + U_r) * (N * s_B + s_sc) + M * N * 4
     k_gbs = kbytes / (spmm_ms * 1e-3) / 1e9
     log(f"step {ms:.3f} ms (pack {pack_ms:.3f} + spmm {spmm_ms:.3f}); {value:.0f} GB/s alg; "
-        f"spmm kernel {k_gbs:.0f} GB/s; gathered rows {nnz_p * N * 4 / (spmm_ms * 1e-3) / 1e9:.0f} GB/s")
+        f"spmm kernel {k_gbs:.0f} GB/s; gathered rows {nnz_p * N * s_B / (spmm_ms * 1e-3) / 1e9:.0f} GB/s")
 
     traffic = prof.get("hbm_bytes_per_launch") if prof is not None and prof.get("tuning") == tuning else None
 
@@ -397,8 +414,8 @@ def main():
     if not args.no_extras:
         Bh = B.cpu().numpy()
         sample = Sample(orc, torch, rp, ci, va, lb, M, Bh, n)
-        extras["fp32_errors_sampled_rows"] = sample.errors(cola, C)
-        log(f"fp32 errors {extras['fp32_errors_sampled_rows']}")
+        extras["%s_errors_sampled_rows" % args.dtype] = sample.errors(cola, C)
+        log(f"{args.dtype} errors {extras['%s_errors_sampled_rows' % args.dtype]}")
         del Bh, sample
         if not args.no_precision:
             extras["precision"] = precision_section(cola, torch, orc, log, (plan, P, M, n, rp, ci, va, lb), args,
@@ -417,14 +434,16 @@ def main():
             zr = cola.d_csc_zip(pr, a["csc_rowIdx_r"], tr, a["csc_val_r"])
             Cr = torch.zeros_like(C)
             # The reference's single-stage wrapper (plan cached from the zips)
-            # and the literal ppl0_batch_SpMM restatement (virtual rows, atomics).
-            ms_ref = timed(torch, lambda: cola.dist_spmm_ppl_batch(Cr, B, packed, zl, zr, N, P, args.rank),
+            # and the literal ppl0_batch_SpMM restatement (virtual rows, atomics),
+            # both FP32 as the reference is.
+            packed32 = torch.empty((max(zr.n_unique, 1), N), dtype=torch.float32, device=dev)
+            ms_ref = timed(torch, lambda: cola.dist_spmm_ppl_batch(Cr, B, packed32, zl, zr, N, P, args.rank),
                            max(3, args.steps // 4), warm=2)
             uniq = torch.from_numpy(np.unique(tr).astype(np.int32)).to(dev)
 
             def literal():
-                cola.pack_remote(B, uniq, packed, num_cols=zr.n_unique)
-                cola.ppl0_batch_SpMM(Cr, B_local, packed, zl, zr, 0, 0, zl.num, zr.num, N, P, args.rank)
+                cola.pack_remote(B, uniq, packed32, num_cols=zr.n_unique)
+                cola.ppl0_batch_SpMM(Cr, B[lb:lb + P], packed32, zl, zr, 0, 0, zl.num, zr.num, N, P, args.rank)
 
             ms_lit = timed(torch, literal, max(3, args.steps // 4), warm=2)
             extras["reference_api_wrapper"] = {"ms_per_step": ms_ref, "GBps_alg": balg / (ms_ref * 1e-3) / 1e9,
@@ -433,7 +452,7 @@ def main():
                 "ms_per_step": ms_lit, "GBps_alg": balg / (ms_lit * 1e-3) / 1e9,
                 "path": "pack + cola_ppl0_batch_SpMM: nnz_split virtual rows, FP32 atomics"}
             log(f"reference API wrapper {ms_ref:.3f} ms; literal ppl0 restatement (atomics) {ms_lit:.3f} ms")
-            del Cr, zl, zr, uniq
+            del Cr, zl, zr, uniq, packed32
         dim_tunings = {}
         if not args.no_batched:
             extras["batched"], dim_tunings = batched_section(cola, torch, log)
@@ -455,10 +474,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": args.dtype,
         "data": "synthetic (seeded R-MAT graph, uniform[-1,1] B generated on the GPU)",
         "config": {"workload": f"rmat{args.log_n}_deg{int(args.degree)}_cm_part{args.rank}of{args.nparts}"
-                               f"_dim{N}_f32",
+                               f"_dim{N}_{args.dtype}",
                    "nodes": n, "nnz_graph": nnz_total, "rows": M, "nnz_partition": nnz_p,
                    "unique_local_cols": U_l, "unique_remote_cols": U_r, "dim": N,
                    "bytes_alg": balg, "step": "pack + plan row kernel + hub reduce, C overwritten",
