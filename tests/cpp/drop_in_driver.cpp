@@ -5,7 +5,12 @@
 // run the single-stage wrapper, and check C[i,:] = deg(i) with the
 // reference's 1e-2 relative test (utils.cuh:71-100).
 //
-// usage: drop_in_driver <bin_path> <graph> <num_GPUs> <rank> <dim>
+// With a sixth argument N, it then times N more calls after 10 warm-up
+// calls with HIP events and prints the average, as the reference's harness
+// does (run_distSpMM_ppl_batch, dist_spmm_ppl_batch.cuh:920-988; C keeps
+// accumulating, as there).
+//
+// usage: drop_in_driver <bin_path> <graph> <num_GPUs> <rank> <dim> [N]
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -88,6 +93,26 @@ int main(int argc, char **argv) {
       return 1;
     }
     printf("PEid-%d passed\n", rank);
+    const int runs = argc > 6 ? atoi(argv[6]) : 0;
+    if (runs > 0) {
+      for (int r = 0; r < 10; ++r)
+        cola_throw_if(cola_dist_spmm_ppl_batch(d_out, d_in, d_packed, csr_zip_l->get(), csc_zip_r->get(), dim, P,
+                                               rank, stream));
+      hipEvent_t e0, e1;
+      if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        throw cola_error(COLA_ERR_HIP, "hipEventCreate");
+      (void)hipEventRecord(e0, stream);
+      for (int r = 0; r < runs; ++r)
+        cola_throw_if(cola_dist_spmm_ppl_batch(d_out, d_in, d_packed, csr_zip_l->get(), csc_zip_r->get(), dim, P,
+                                               rank, stream));
+      (void)hipEventRecord(e1, stream);
+      if (hipEventSynchronize(e1) != hipSuccess) throw cola_error(COLA_ERR_HIP, "hipEventSynchronize");
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      printf("PEid-%d average time over %d runs: %.4f ms\n", rank, runs, ms / runs);
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+    }
     delete csr_zip_l;
     delete csc_zip_r;
     (void)hipFree(d_in);

@@ -24,3 +24,16 @@ def test_cpp_driver_known_answer(tmp_path, dim):
                              text=True, timeout=60)
         assert out.returncode == 0, out.stdout + out.stderr
         assert f"PEid-{rank} passed" in out.stdout
+
+
+def test_cpp_driver_timing_loop(tmp_path):
+    """The harness form (10 warm-up + N timed calls, C accumulating) runs and
+    reports an average time."""
+    from tests.cpp_driver import build_driver
+    exe = build_driver(tmp_path)
+    g = cola.Graph.rmat(13, 12, seed=21, reorder=True, gcn=False)
+    cola.serialize(g, str(tmp_path), "rmat13", [2], write_values=False)
+    out = subprocess.run([exe, str(tmp_path), "rmat13", "2", "0", "64", "20"], capture_output=True, text=True,
+                         timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "PEid-0 passed" in out.stdout and "average time over 20 runs" in out.stdout
