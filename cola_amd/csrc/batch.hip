@@ -113,12 +113,12 @@ __global__ __launch_bounds__(256) void batch_rows_kernel(const BatchSeg *__restr
     // The lane-group width is wave-uniform (one segment per task): each case
     // is the plan kernel's step with that width compiled in.
     switch (sg.g_log2) {
-      case 1: cola_dev::row_step<DT, EPL, 2, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
-      case 2: cola_dev::row_step<DT, EPL, 4, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
-      case 3: cola_dev::row_step<DT, EPL, 8, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
-      case 4: cola_dev::row_step<DT, EPL, 16, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
-      case 5: cola_dev::row_step<DT, EPL, 32, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
-      default: cola_dev::row_step<DT, EPL, 64, 1, 8, KAHAN>(a, it0, sg.n_items, sg.rpw); break;
+      case 1: cola_dev::row_step<DT, EPL, 2, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
+      case 2: cola_dev::row_step<DT, EPL, 4, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
+      case 3: cola_dev::row_step<DT, EPL, 8, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
+      case 4: cola_dev::row_step<DT, EPL, 16, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
+      case 5: cola_dev::row_step<DT, EPL, 32, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
+      default: cola_dev::row_step<DT, EPL, 64, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
     }
   }
 }
@@ -137,10 +137,12 @@ __global__ __launch_bounds__(256) void batch_hub_reduce_kernel(const HubSeg *__r
   const HubSeg &h = hs[lo];
   const int64_t r = t - h.thread_begin;
   const int hub = (int)(r / h.dim), col = (int)(r % h.dim);
-  const int c0 = h.hub_ptr[hub], c1 = h.hub_ptr[hub + 1];
+  // The table's pointers are read from memory: gld / gst keep the accesses
+  // global rather than flat.
+  const int c0 = cola_dev::gld(h.hub_ptr + hub), c1 = cola_dev::gld(h.hub_ptr + hub + 1);
   float s = 0.f, err = 0.f;
   for (int c = c0; c < c1; ++c) {
-    const float x = h.partials[(int64_t)c * h.pstride + col];
+    const float x = cola_dev::gld(h.partials + (int64_t)c * h.pstride + col);
     if (kahan) {
       const float y = x - err;
       const float tt = s + y;
@@ -150,8 +152,8 @@ __global__ __launch_bounds__(256) void batch_hub_reduce_kernel(const HubSeg *__r
       s += x;
     }
   }
-  float *dst = h.C + (int64_t)h.hub_rows[hub] * h.ldC + col;
-  *dst = accumulate ? *dst + s : s;
+  float *dst = h.C + (int64_t)cola_dev::gld(h.hub_rows + hub) * h.ldC + col;
+  cola_dev::gst(dst, accumulate ? cola_dev::gld(dst) + s : s);
 }
 
 template <int DT, int EPL>

@@ -12,8 +12,9 @@
 // taking every (S/rpw)-th entry, U loads in flight per group, and the groups
 // of a team are summed with cross-lane shuffles at the end.  rpw = 1 is one
 // item per wave (long rows); rpw = S is one item per group (short rows).
-// Entry (column, value) pairs are staged K per team lane (K = U/G when U > G,
-// else 1) with coalesced loads and broadcast with ds_bpermute.
+// Entries are staged K per team lane (K = U/G when U > G, else 1) with
+// coalesced loads, turned into (B row address, value) pairs there, and
+// broadcast with ds_bpermute.
 #pragma once
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
@@ -36,8 +37,49 @@ __device__ __forceinline__ float fp8e4m3_to_f32(uint32_t c) {
   return (c & 0x80u) ? -mag : mag;
 }
 
+// Load of type X from a global-memory address held as an integer (the row
+// kernel stages row addresses as integers; the address space keeps the load
+// a global_load rather than a flat one, which would also count against the
+// LDS wait counter the broadcasts use).
+template <typename X>
+__device__ __forceinline__ X load_global(uint64_t addr) {
+  return *reinterpret_cast<const __attribute__((address_space(1))) X *>(addr);
+}
+// 16-byte forms as clang vector types (HIP's float4 / uint4 structs do not
+// copy from an address-space-qualified lvalue).
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 load_global_f4(uint64_t addr) {
+  const f32x4_t v = load_global<f32x4_t>(addr);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 load_global_u4(uint64_t addr) {
+  const u32x4_t v = load_global<u32x4_t>(addr);
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+// Pointer forms, for pointers the compiler cannot prove global (those the
+// batched kernel reads from its segment table in memory).
+template <typename X>
+__device__ __forceinline__ X gld(const X *p) {
+  return load_global<X>(reinterpret_cast<uint64_t>(p));
+}
+__device__ __forceinline__ int4 gld(const int4 *p) {
+  typedef int32_t i32x4_t __attribute__((ext_vector_type(4)));
+  const i32x4_t v = load_global<i32x4_t>(reinterpret_cast<uint64_t>(p));
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 gld(const float4 *p) { return load_global_f4(reinterpret_cast<uint64_t>(p)); }
+__device__ __forceinline__ void gst(float *p, float v) {
+  *reinterpret_cast<__attribute__((address_space(1))) float *>(reinterpret_cast<uint64_t>(p)) = v;
+}
+__device__ __forceinline__ void gst(float4 *p, const float4 &v) {
+  const f32x4_t w = {v.x, v.y, v.z, v.w};
+  *reinterpret_cast<__attribute__((address_space(1))) f32x4_t *>(reinterpret_cast<uint64_t>(p)) = w;
+}
+
 // Vec<DT, EPL>: element type T, the raw register form of one lane's vector,
-// its load and its widening to FP32.
+// its load (from a pointer, or from a global address: gload) and its
+// widening to FP32.
 template <int DT, int EPL>
 struct Vec;
 template <>
@@ -45,6 +87,7 @@ struct Vec<COLA_F32, 4> {
   using T = float;
   using Raw = float4;
   static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const float4 *>(p); }
+  static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global_f4(a); }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[4]) {
     x[0] = r.x; x[1] = r.y; x[2] = r.z; x[3] = r.w;
   }
@@ -54,6 +97,7 @@ struct Vec<COLA_F32, 1> {
   using T = float;
   using Raw = float;
   static __device__ __forceinline__ Raw load(const T *p) { return *p; }
+  static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global<float>(a); }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = r; }
 };
 template <int DT>
@@ -61,6 +105,7 @@ struct Vec16 {
   using T = uint16_t;
   using Raw = uint4;
   static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const uint4 *>(p); }
+  static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global_u4(a); }
   static __device__ __forceinline__ float cvt(uint32_t h) {
     if constexpr (DT == COLA_BF16) return bf16_to_f32(h);
     else return f16_to_f32(h);
@@ -83,6 +128,7 @@ struct Vec16x1 {
   using T = uint16_t;
   using Raw = uint32_t;
   static __device__ __forceinline__ Raw load(const T *p) { return *p; }
+  static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global<uint16_t>(a); }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = Vec16<DT>::cvt(r); }
 };
 template <>
@@ -94,6 +140,7 @@ struct Vec<COLA_FP8_E4M3, 16> {
   using T = uint8_t;
   using Raw = uint4;
   static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const uint4 *>(p); }
+  static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global_u4(a); }
   // gfx950's v_cvt_pk_f32_fp8 decodes OCP E4M3 (checked against the format
   // definition for all 256 codes), two bytes per instruction.
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[16]) {
@@ -114,6 +161,7 @@ struct Vec<COLA_FP8_E4M3, 1> {
   using T = uint8_t;
   using Raw = uint32_t;
   static __device__ __forceinline__ Raw load(const T *p) { return *p; }
+  static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global<uint8_t>(a); }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = fp8e4m3_to_f32(r); }
 };
 
@@ -142,11 +190,24 @@ struct RowArgs {
                   // added to what the first pass of a hot/cold plan wrote
 };
 
+// Two forms of the staging (PIPE): false turns the next chunk's staged
+// entries into row addresses as soon as they arrive; true defers that (and
+// FP8's scale loads) until this chunk's row loads are in flight and keeps the
+// compiler from interleaving loads with their use.  Chosen per kernel by
+// measurement: on the 2^22-node partition at dim 128 the plan kernel is
+// 1.6% (FP32), 2.6% (BF16) and 3% (FP8) faster with PIPE, each at its best
+// schedule, and far less sensitive to the schedule; the batched kernel (64
+// BF16 problems) is 3% faster without.
+template <int DT>
+inline constexpr bool kPlanPipe = true;
+template <int DT>
+inline constexpr bool kBatchPipe = false;
+
 // One step of a wave: the `rpw` items it0 .. it0+rpw-1 (those below ie), one
 // per team, over the column range [0, a.dim) of the tile a's pointers start
 // at.  Shared by the plan kernel (compile-time G per launch) and the batched
 // persistent kernel (G chosen per segment).
-template <int DT, int EPL, int G, int NV, int U, bool KAHAN>
+template <int DT, int EPL, int G, int NV, int U, bool KAHAN, bool PIPE>
 __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int rpw) {
   using V = Vec<DT, EPL>;
   using T = typename V::T;
@@ -163,7 +224,7 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
   const int it = it0 + team;
   const bool has = it < ie;
   int4 item = make_int4(0, 0, 0, 0);
-  if (has) item = a.items[it];
+  if (has) item = gld(a.items + it);
   int b = item.y, e = item.z;
   if (a.phase == 1) e = b + (item.w >> 1);
   else if (a.phase == 2) b += item.w >> 1;
@@ -175,65 +236,112 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
 #pragma unroll
     for (int q = 0; q < EPL; ++q) acc[k][q] = err[k][q] = 0.f;
 
-  // Entries are taken in chunks of K*tsize: each team lane stages K
-  // (column, value) pairs with coalesced loads, the next chunk prefetched
-  // while this one is used.  A lane whose entry lies past the item's end
-  // stages the last entry's column with value 0, so every broadcast slot
-  // holds a valid row address and no load or FMA is predicated.
-  constexpr int K = U > G ? U / G : 1;  // pairs per lane: one batch of U per group fills K*tsize
+  // Entries are taken in chunks of K*tsize: each team lane stages K entries
+  // of the next chunk while this one is used, in two steps: the coalesced
+  // (column, value) loads are issued before this chunk's row loads, and
+  // turned into the address of their B row (local slice or packed buffer)
+  // after them, so the wait for them overlaps the row loads in flight.  The
+  // address arithmetic thus runs once per entry rather than once per
+  // broadcast slot.  FP8 rows also take their scale there, loaded after the
+  // row loads and multiplied into the value at the end of the chunk.  A lane
+  // whose entry lies past the item's end stages the last entry with value 0,
+  // so every broadcast slot holds a valid row address and no load or FMA is
+  // predicated.
+  constexpr int K = U > G ? U / G : 1;  // entries per lane: one batch of U per group fills K*tsize
   const int csize = K * tsize;
-  int mc[K], mc_next[K];
-  float mv[K], mv_next[K];
-  auto stage = [&](int base, int (&c)[K], float (&v)[K]) {
+  uint32_t mlo[K], mhi[K], mlo_next[K], mhi_next[K];
+  float mv[K], mv_next[K], msc_next[K];
+  int nc[K];    // staged columns of the next chunk
+  float nx[K];  // staged values of the next chunk
+  auto stage_load = [&](int base) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const int idx = base + k * tsize + tl;
-      const int cl = min(idx, e - 1);
-      c[k] = a.cols[cl];
-      const float x = a.vals[cl];
-      v[k] = idx < e ? x : 0.f;
+      const int cl = min(base + k * tsize + tl, e - 1);
+      nc[k] = gld(a.cols + cl);
+      nx[k] = gld(a.vals + cl);
+    }
+  };
+  auto stage_addr = [&](int base) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int c = nc[k];
+      const bool loc = c >= 0;
+      const T *rp = loc ? Bl + (int64_t)c * a.ldB : Bp + (int64_t)(~c) * a.ldP;
+      const uint64_t p = reinterpret_cast<uint64_t>(rp);
+      mlo_next[k] = (uint32_t)p;
+      mhi_next[k] = (uint32_t)(p >> 32);
+      mv_next[k] = base + k * tsize + tl < e ? nx[k] : 0.f;
+      if constexpr (DT == COLA_FP8_E4M3) msc_next[k] = gld(loc ? a.sl + c : a.sp + ~c);
+    }
+  };
+  auto rotate = [&]() {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      mlo[k] = mlo_next[k];
+      mhi[k] = mhi_next[k];
+      mv[k] = DT == COLA_FP8_E4M3 ? mv_next[k] * msc_next[k] : mv_next[k];
     }
   };
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    mc[k] = mc_next[k] = 0;
+    mlo[k] = mhi[k] = mlo_next[k] = mhi_next[k] = 0;
     mv[k] = mv_next[k] = 0.f;
+    msc_next[k] = 1.f;
+    nc[k] = 0;
+    nx[k] = 0.f;
   }
-  if (b < e) stage(b, mc, mv);
+  // This lane's offset into a B row, per vector (clamped to the row start
+  // for the lanes past dim, whose results are never stored).
+  int lcol[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const int col = (k * G + gl) * EPL;
+    lcol[k] = col < a.dim ? col : 0;
+  }
+  if (b < e) {
+    stage_load(b);
+    stage_addr(b);
+    rotate();
+  }
   for (int base = b; base < e; base += csize) {
     const int n = min(csize, e - base);
-    if (base + csize < e) stage(base + csize, mc_next, mv_next);
+    const bool more = base + csize < e;
+    if (more) stage_load(base + csize);
+    if constexpr (!PIPE)
+      if (more) stage_addr(base + csize);
     for (int j = 0; j < n; j += gpt * U) {
-      // Broadcast the batch's (column, value) pairs first, then issue every
-      // load.  With K > 1 the batch is the whole chunk (j = 0) and slot u
-      // reads pair u / G of team lane (u % G) * gpt + tsub.
-      int cu[U];
-      float vu[U];
+      // Broadcast the batch's (row address, value) pairs first, then issue
+      // every load.  With K > 1 the batch is the whole chunk (j = 0) and
+      // slot u reads entry u / G of team lane (u % G) * gpt + tsub.
+      uint64_t pu[U];
+      float av[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int k = K > 1 ? u / G : 0;
         const int src = (tbase + (K > 1 ? (u % G) * gpt + tsub : j + u * gpt + tsub)) << 2;
-        cu[u] = __builtin_amdgcn_ds_bpermute(src, mc[k]);
-        vu[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv[k])));
+        const uint32_t lo = __builtin_amdgcn_ds_bpermute(src, (int)mlo[k]);
+        const uint32_t hi = __builtin_amdgcn_ds_bpermute(src, (int)mhi[k]);
+        pu[u] = ((uint64_t)hi << 32) | lo;
+        av[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv[k])));
       }
       Raw raw[U][NV];
-      float av[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int c = cu[u];
-        const bool loc = c >= 0;
-        const T *rp = (loc ? Bl : Bp) + (int64_t)(loc ? c : ~c) * (loc ? a.ldB : a.ldP);
+      for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int k = 0; k < NV; ++k) {
-          const int col = (k * G + gl) * EPL;
-          raw[u][k] = V::load(rp + (col < a.dim ? col : 0));
+        for (int k = 0; k < NV; ++k) raw[u][k] = V::gload(pu[u] + (uint64_t)lcol[k] * sizeof(T));
+      if constexpr (PIPE) {
+        if (more && j == 0) {
+          // Pin the staged registers here, after this batch's row loads (the
+          // memory clobber keeps the loads above), so the address arithmetic
+          // and its wait for the staged loads are not hoisted above them.
+#pragma unroll
+          for (int k = 0; k < K; ++k) asm volatile("" : "+v"(nc[k]), "+v"(nx[k])::"memory");
+          stage_addr(base + csize);
         }
-      }
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        float v = vu[u];
-        if constexpr (DT == COLA_FP8_E4M3) v *= (cu[u] >= 0 ? a.sl[cu[u]] : a.sp[~cu[u]]);
-        av[u] = v;
+        // Keep every load of the batch ahead of its use: without this the
+        // scheduler interleaves loads with the FP8 / 16-bit widening to save
+        // registers and leaves one or two loads in flight instead of U.
+        __builtin_amdgcn_sched_barrier(0);
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -256,11 +364,7 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
           }
         }
     }
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      mc[k] = mc_next[k];
-      mv[k] = mv_next[k];
-    }
+    rotate();
   }
   // Sum the groups of each team (xor offsets below tsize stay in the team).
   for (int off = G; off < tsize; off <<= 1) {
@@ -305,14 +409,14 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
             float4 *d4 = reinterpret_cast<float4 *>(dst + col + q);
             float4 o = make_float4(v[q], v[q + 1], v[q + 2], v[q + 3]);
             if (add_old) {
-              const float4 old = *d4;
+              const float4 old = gld(d4);
               o.x += old.x; o.y += old.y; o.z += old.z; o.w += old.w;
             }
-            *d4 = o;
+            gst(d4, o);
           }
         } else {
 #pragma unroll
-          for (int q = 0; q < EPL; ++q) dst[col + q] = add_old ? dst[col + q] + v[q] : v[q];
+          for (int q = 0; q < EPL; ++q) gst(dst + col + q, add_old ? gld(dst + col + q) + v[q] : v[q]);
         }
       }
     }
@@ -339,7 +443,8 @@ __global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
   }
   const int wave = blk * wpb + wid;
   const int nwave = nblk * wpb;
-  for (int it0 = ib + wave * rpw; it0 < ie; it0 += nwave * rpw) row_step<DT, EPL, G, NV, U, KAHAN>(a, it0, ie, rpw);
+  for (int it0 = ib + wave * rpw; it0 < ie; it0 += nwave * rpw)
+    row_step<DT, EPL, G, NV, U, KAHAN, kPlanPipe<DT>>(a, it0, ie, rpw);
 }
 
 // Lanes per row group and vectors per lane for a dim.  vec_ok: dim, leading

@@ -830,9 +830,10 @@ static cola_status plan_from_partition(const cola::Partition &part, const cola_p
   p->pstride = (hp.max_dim + 15) / 16 * 16;
   p->nnz = hp.nnz;
   // Default schedule by size: large partitions gather from far more B rows
-  // than the L2s hold and want 128-byte column tiles at low occupancy with 16
-  // loads in flight; small ones want every column in one pass and many waves.
-  p->tune = hp.nnz >= (int64_t(1) << 22) ? cola_plan_tuning{-1, 0, 4, 2, 0, 0, 16, 0}
+  // than the L2s hold and want 128-byte column tiles at low occupancy (one
+  // block per CU, 8 rows per wave) with 16 loads in flight; small ones want
+  // every column in one pass and many waves.
+  p->tune = hp.nnz >= (int64_t(1) << 22) ? cola_plan_tuning{-1, 0, 8, 1, 0, 0, 16, 0}
                                          : cola_plan_tuning{0, 0, 1, 8, 1, 0, 8, 0};
   p->num_empty = (int64_t)hp.empty_rows.size();
   p->num_hubs = (int64_t)hp.hub_rows.size();

@@ -319,8 +319,8 @@ typedef struct {
 cola_status cola_plan_get_info(cola_plan plan, cola_plan_info *info);
 
 /* Launch parameters of the plan's kernels (the autotuner's search space).
- * A new plan with at least 2^22 entries starts with {-1, 0, 4, 2, 0, 0, 16,
- * 0}: 128-byte column tiles, 4 rows per wave on long rows, 2 blocks per CU,
+ * A new plan with at least 2^22 entries starts with {-1, 0, 8, 1, 0, 0, 16,
+ * 0}: 128-byte column tiles, 8 rows per wave on long rows, 1 block per CU,
  * 16 loads in flight (the tuned schedule of the 2^22-node partition); a
  * smaller one with {0, 0, 1, 8, 1, 0, 8, 0}: all columns in one pass, one
  * row per wave on long rows, 8 blocks per CU. */
