@@ -23,8 +23,9 @@
 // SpMM is HBM/cache-bandwidth bound (about 2 FLOP per 8+ bytes), so nothing
 // here uses MFMA.  The dense dimension is spread across the lanes of a row
 // group (G lanes, 16 bytes per lane per vector), several row groups share a
-// wave, and (column, value) pairs are staged per team lane with coalesced
-// loads and broadcast to the groups with ds_bpermute (rows.hpp).
+// wave, and entries are staged per team lane with coalesced loads, turned
+// into (B row address, value) pairs there and broadcast to the groups with
+// ds_bpermute (rows.hpp).
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 

@@ -512,8 +512,10 @@ TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=
            dict(col_tile=32, rpw_short=0, rpw_long=2, blocks_per_cu=4, xcd_remap=0, split_remote=0, unroll=8,
                 panels=1),
            # 16 loads in flight on groups narrower than 16 lanes: each lane
-           # stages 2, 4 or 8 (column, value) pairs per chunk
+           # stages 2, 4 or 8 entries per chunk
            dict(col_tile=32, rpw_short=8, rpw_long=8, blocks_per_cu=2, xcd_remap=0, split_remote=0, unroll=16),
+           # the large-graph default: 128-byte tiles, 1 block per CU
+           dict(col_tile=-1, rpw_short=0, rpw_long=8, blocks_per_cu=1, xcd_remap=0, split_remote=0, unroll=16),
            dict(col_tile=16, rpw_short=16, rpw_long=8, blocks_per_cu=3, xcd_remap=1, split_remote=1, unroll=16),
            dict(col_tile=8, rpw_short=32, rpw_long=4, blocks_per_cu=2, xcd_remap=0, split_remote=0, unroll=16)]
 
