@@ -2,9 +2,10 @@
 
 Workload (the real shape of one partition): R-MAT graph with 2^22 nodes,
 average degree 32, a = 0.57, b = c = 0.19, symmetrised, self-loops and
-duplicates dropped, GCN-normalised values, Cuthill-McKee locality order;
-8 row partitions of 2^19 rows, partition 0 (the heaviest one, which sets a
-distributed step's critical path); B is 2^22 x 128 FP32 uniform[-1, 1],
+duplicates dropped, GCN-normalised values, relabelled by the community
+locality order (Rabbit Order's aggregation and depth-first numbering;
+--order cm for Cuthill-McKee); 8 row partitions of 2^19 rows, partition 0;
+B is 2^22 x 128 FP32 uniform[-1, 1],
 resident in HBM.  One step = gather-pack of the unique remote B rows + the
 degree-binned row kernel + the hub reduction, C overwritten.
 
@@ -264,6 +265,8 @@ def main():
     ap.add_argument("--dtype", choices=["f32", "bf16", "f16", "fp8_e4m3"], default="f32",
                     help="storage dtype of B in the main step (the metric is quoted for f32)")
     ap.add_argument("--row-order", type=int, default=1, help="plan item order: 0 row order, 1 by degree")
+    ap.add_argument("--order", choices=["cm", "community"], default="community",
+                    help="locality order of the graph's vertex ids before partitioning")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip every extra section")
@@ -300,7 +303,7 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     t0 = time.time()
-    g = cola.Graph.rmat(args.log_n, args.degree, seed=2, reorder=True)
+    g = cola.Graph.rmat(args.log_n, args.degree, seed=2, reorder=args.order)
     n, nnz_total, _ = g.info()
     part = g.partition(args.nparts, args.rank)
     info = part.info()
@@ -345,12 +348,14 @@ def main():
     pack()
     # The committed counter profile (HBM bytes per launch) holds only for the
     # workload and schedule it was measured with.
-    tfile = os.path.join(ROOT, "profiles", "plan_rows_f32_rmat22_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", {"community": "plan_rows_f32_rmat22_traffic.json",
+                                            "cm": "plan_rows_f32_rmat22cm_traffic.json"}[args.order])
     prof = None
     if os.path.exists(tfile):
         with open(tfile) as f:
             prof = json.load(f)
-        if not (prof.get("row_order", 0) == args.row_order and N == 128 and args.log_n == 22
+        if not (prof.get("row_order", 0) == args.row_order and prof.get("order", "cm") == args.order
+                and N == 128 and args.log_n == 22
                 and args.degree == 32 and args.nparts == 8 and args.rank == 0 and dt == cola.F32):
             prof = None
     tuning_source = args.tuning
@@ -476,7 +481,7 @@ def main():
         "vs_baseline": None,
         "dtype": args.dtype,
         "data": "synthetic (seeded R-MAT graph, uniform[-1,1] B generated on the GPU)",
-        "config": {"workload": f"rmat{args.log_n}_deg{int(args.degree)}_cm_part{args.rank}of{args.nparts}"
+        "config": {"workload": f"rmat{args.log_n}_deg{int(args.degree)}_{args.order}_part{args.rank}of{args.nparts}"
                                f"_dim{N}_{args.dtype}",
                    "nodes": n, "nnz_graph": nnz_total, "rows": M, "nnz_partition": nnz_p,
                    "unique_local_cols": U_l, "unique_remote_cols": U_r, "dim": N,

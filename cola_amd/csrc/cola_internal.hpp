@@ -34,6 +34,9 @@ struct Partition {
 void assign_values(Graph *g, bool gcn);
 // Cuthill-McKee order, new -> old.
 std::vector<int32_t> cuthill_mckee(const Graph &g);
+// Community order (Rabbit Order's aggregation + depth-first numbering,
+// community.cpp), new -> old.
+std::vector<int32_t> community_order(const Graph &g);
 // out = in relabelled (new vertex i = old new2old[i]); values carried when
 // in.val is non-empty; columns sorted within each row.
 void relabel(const Graph &in, const std::vector<int32_t> &new2old, Graph *out);

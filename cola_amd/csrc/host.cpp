@@ -702,7 +702,7 @@ cola_status cola_rmat_graph(int log_n, double avg_degree, double a, double b,
   if (log_n < 1 || log_n > 30) return fail(COLA_ERR_INVALID, "rmat: log_n must be in [1,30]");
   if (!(avg_degree > 0) || a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
     return fail(COLA_ERR_INVALID, "rmat: bad degree or quadrant probabilities");
-  if (reorder != 0 && reorder != 1) return fail(COLA_ERR_INVALID, "rmat: reorder must be 0 or 1");
+  if (reorder < 0 || reorder > 2) return fail(COLA_ERR_INVALID, "rmat: reorder must be 0, 1 or 2");
   const int64_t n = (int64_t)1 << log_n;
   const int64_t m = (int64_t)std::llround((double)n * avg_degree / 2.0);
   const double s32 = 4294967296.0;
@@ -743,9 +743,9 @@ cola_status cola_rmat_graph(int log_n, double avg_degree, double a, double b,
     delete h;
     return fail(COLA_ERR_INVALID, "rmat: nnz overflows int32");
   }
-  if (reorder == 1) {
+  if (reorder != 0) {
     Graph re;
-    relabel(h->g, cuthill_mckee(h->g), &re);
+    relabel(h->g, reorder == 1 ? cuthill_mckee(h->g) : community_order(h->g), &re);
     h->g = std::move(re);
   }
   assign_values(&h->g, gcn_values != 0);

@@ -403,6 +403,9 @@ cola_status cola_graph_reorder(cola_graph g, int method, int32_t *new2old, cola_
       order = cola::cuthill_mckee(G);
       std::reverse(order.begin(), order.end());
       break;
+    case COLA_ORDER_COMMUNITY:
+      order = cola::community_order(G);
+      break;
     case COLA_ORDER_DEGREE: {
       order.resize(n);
       for (int64_t i = 0; i < n; ++i) order[i] = (int32_t)i;

@@ -240,10 +240,11 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
   // of the next chunk while this one is used, in two steps: the coalesced
   // (column, value) loads are issued before this chunk's row loads, and
   // turned into the address of their B row (local slice or packed buffer)
-  // after them, so the wait for them overlaps the row loads in flight.  The
+  // by stage_addr (with PIPE after this chunk's row loads are issued, so the
+  // wait for the staged loads overlaps the row loads in flight).  The
   // address arithmetic thus runs once per entry rather than once per
-  // broadcast slot.  FP8 rows also take their scale there, loaded after the
-  // row loads and multiplied into the value at the end of the chunk.  A lane
+  // broadcast slot.  FP8 rows also take their scale there, multiplied into
+  // the value when the chunk becomes current (rotate).  A lane
   // whose entry lies past the item's end stages the last entry with value 0,
   // so every broadcast slot holds a valid row address and no load or FMA is
   // predicated.

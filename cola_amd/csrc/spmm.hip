@@ -476,7 +476,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
                       aligned(c.C, 16) && aligned(c.Bl, 16) && (p->U_r == 0 || aligned(c.Bp, 16));
   // Column tile: a multiple of the vector width (so tiles stay aligned).
   int tile = c.dim;
-  const int want = t.col_tile < 0 ? 128 / dtype_bytes(c.dtype) : t.col_tile;
+  const int want = t.col_tile < 0 ? -t.col_tile * 128 / dtype_bytes(c.dtype) : t.col_tile;
   if (want > 0 && want < c.dim && vec_ok && c.dim % ev == 0) {
     tile = std::max(ev, want / ev * ev);
   }
@@ -831,10 +831,10 @@ static cola_status plan_from_partition(const cola::Partition &part, const cola_p
   p->pstride = (hp.max_dim + 15) / 16 * 16;
   p->nnz = hp.nnz;
   // Default schedule by size: large partitions gather from far more B rows
-  // than the L2s hold and want 128-byte column tiles at low occupancy (one
-  // block per CU, 8 rows per wave) with 16 loads in flight; small ones want
-  // every column in one pass and many waves.
-  p->tune = hp.nnz >= (int64_t(1) << 22) ? cola_plan_tuning{-1, 0, 8, 1, 0, 0, 16, 0}
+  // than the L2s hold and want 256-byte column tiles, 4 rows per wave at 2
+  // blocks per CU and each XCD on its own item range; small ones want every
+  // column in one pass and many waves.
+  p->tune = hp.nnz >= (int64_t(1) << 22) ? cola_plan_tuning{-2, 0, 4, 2, 1, 0, 8, 0}
                                          : cola_plan_tuning{0, 0, 1, 8, 1, 0, 8, 0};
   p->num_empty = (int64_t)hp.empty_rows.size();
   p->num_hubs = (int64_t)hp.hub_rows.size();
@@ -1001,7 +1001,7 @@ cola_status cola_plan_get_info(cola_plan p, cola_plan_info *info) {
 
 cola_status cola_plan_set_tuning(cola_plan p, const cola_plan_tuning *t) {
   if (!p || !t) return fail(COLA_ERR_INVALID, "plan_set_tuning: NULL argument");
-  if (t->col_tile < -1 || t->rpw_short < 0 || t->rpw_short > 64 || t->rpw_long < 0 || t->rpw_long > 64 ||
+  if (t->col_tile < -8 || t->rpw_short < 0 || t->rpw_short > 64 || t->rpw_long < 0 || t->rpw_long > 64 ||
       t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || (t->xcd_remap != 0 && t->xcd_remap != 1) ||
       (t->split_remote != 0 && t->split_remote != 1) || (t->unroll != 0 && t->unroll != 8 && t->unroll != 16) ||
       (t->panels != 0 && t->panels != 1))

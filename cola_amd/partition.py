@@ -47,10 +47,14 @@ class Graph:
     def rmat(cls, log_n, avg_degree, a=0.57, b=0.19, c=0.19, seed=1, reorder=False, gcn=True):
         """Seeded R-MAT graph, symmetrised, self-loops and duplicates dropped,
         GCN-normalised values 1/sqrt(d_i d_j) (or 1.0 when gcn=False);
-        reorder=True applies the Cuthill-McKee locality order."""
+        reorder=True (or "cm") applies the Cuthill-McKee locality order,
+        "community" the community order (Rabbit Order's two steps)."""
+        codes = {False: 0, None: 0, "none": 0, True: 1, "cm": 1, "community": 2}
+        if reorder not in codes:
+            raise ValueError(f"rmat: reorder must be one of {list(codes)}, got {reorder!r}")
         h = ctypes.c_void_p()
         check(lib().cola_rmat_graph(int(log_n), float(avg_degree), float(a), float(b), float(c),
-                                    int(seed), 1 if reorder else 0, 1 if gcn else 0, ctypes.byref(h)))
+                                    int(seed), codes[reorder], 1 if gcn else 0, ctypes.byref(h)))
         return cls(h)
 
     @classmethod
@@ -92,8 +96,9 @@ class Graph:
 
     def reorder(self, method="cm"):
         """Relabelled copy and its new->old permutation.  method: "cm"
-        (Cuthill-McKee), "rcm" (reversed) or "degree" (degree descending)."""
-        m = {"cm": 1, "rcm": 2, "degree": 3}[method]
+        (Cuthill-McKee), "rcm" (reversed), "degree" (degree descending) or
+        "community" (Rabbit Order's aggregation and depth-first numbering)."""
+        m = {"cm": 1, "rcm": 2, "degree": 3, "community": 4}[method]
         perm = np.empty(self.n, dtype=np.int32)
         h = ctypes.c_void_p()
         check(lib().cola_graph_reorder(self._h, m, _ptr(perm), ctypes.byref(h)))

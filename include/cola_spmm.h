@@ -74,7 +74,8 @@ typedef struct cola_graph_s *cola_graph;
  * with quadrant probabilities (a, b, c, 1-a-b-c), each edge added in both
  * directions, self-loops and duplicates dropped, values 1/sqrt(d_i·d_j)
  * (GCN normalisation) when gcn_values != 0, else 1.0.
- * reorder: 0 = natural R-MAT ids, 1 = Cuthill-McKee locality order.
+ * reorder: 0 = natural R-MAT ids, 1 = Cuthill-McKee locality order,
+ * 2 = community order (COLA_ORDER_COMMUNITY).
  * Deterministic for a given seed whatever the host thread count. */
 cola_status cola_rmat_graph(int log_n, double avg_degree, double a, double b,
                             double c, uint64_t seed, int reorder,
@@ -127,7 +128,12 @@ enum {
   COLA_ORDER_CUTHILL_MCKEE = 1, /* BFS from min-degree vertices, neighbours
                                    by (degree, id); isolated vertices last */
   COLA_ORDER_REVERSE_CM = 2,    /* the Cuthill-McKee order reversed         */
-  COLA_ORDER_DEGREE = 3         /* degree descending, ties by id           */
+  COLA_ORDER_DEGREE = 3,        /* degree descending, ties by id           */
+  COLA_ORDER_COMMUNITY = 4      /* Rabbit Order's two steps (incremental
+                                   modularity aggregation in increasing
+                                   degree order, then a depth-first walk of
+                                   the merge trees): each community gets a
+                                   contiguous id range; isolated last      */
 };
 
 /* Relabels the graph (values carried along): new vertex i = old vertex
@@ -319,15 +325,17 @@ typedef struct {
 cola_status cola_plan_get_info(cola_plan plan, cola_plan_info *info);
 
 /* Launch parameters of the plan's kernels (the autotuner's search space).
- * A new plan with at least 2^22 entries starts with {-1, 0, 8, 1, 0, 0, 16,
- * 0}: 128-byte column tiles, 8 rows per wave on long rows, 1 block per CU,
- * 16 loads in flight (the tuned schedule of the 2^22-node partition); a
+ * A new plan with at least 2^22 entries starts with {-2, 0, 4, 2, 1, 0, 8,
+ * 0}: 256-byte column tiles, 4 rows per wave on long rows, 2 blocks per CU,
+ * XCD remap, 8 loads in flight (the tuned schedule of a partition of the
+ * community-ordered 2^22-node R-MAT graph); a
  * smaller one with {0, 0, 1, 8, 1, 0, 8, 0}: all columns in one pass, one
  * row per wave on long rows, 8 blocks per CU. */
 typedef struct {
   int32_t col_tile;      /* dense columns per pass over B (0 = all of dim;
-                            -1 = 128 bytes of the storage dtype); rounded
-                            down to the 16-byte vector width                */
+                            -k, k = 1..8: k x 128 bytes of the storage
+                            dtype); rounded down to the 16-byte vector
+                            width                                           */
   int32_t rpw_short;     /* short-row bin: rows in flight per wave (power of
                             two; 0 = one row per lane group)                */
   int32_t rpw_long;      /* long-row bin: rows in flight per wave (0 = 1)   */

@@ -5,7 +5,8 @@ root: python tests/golden/make_fixtures.py).
   symmetrised, GCN values): total nnz, max degree, a CRC32 of the CSR arrays
   and, per row partition, nnz_l, nnz_r, L, U_r, U_l, rows.  C1 = 2^16 nodes,
   degree 16, seed 1, 4 partitions; C2 = 2^22 nodes, degree 32, seed 2,
-  Cuthill-McKee order, 8 partitions (the benchmarked workload).
+  community order, 8 partitions (the benchmarked workload); C2_cm = the same
+  graph in Cuthill-McKee order.
 - par_small.mtx + par_small.npz: a small MatrixMarket graph (duplicates, a
   hub row, a symmetric half) and the int32 partition files the reference's
   writer produces for it (prep_torch_par.py:14-101, restated with the same
@@ -42,8 +43,10 @@ def describe(g, nGPU):
 def main():
     configs = {
         "C1": dict(describe(cola.Graph.rmat(16, 16, seed=1), 4), log_n=16, degree=16, seed=1, reorder=False, nGPU=4),
-        "C2": dict(describe(cola.Graph.rmat(22, 32, seed=2, reorder=True), 8), log_n=22, degree=32, seed=2,
-                   reorder=True, nGPU=8),
+        "C2": dict(describe(cola.Graph.rmat(22, 32, seed=2, reorder="community"), 8), log_n=22, degree=32,
+                   seed=2, reorder="community", nGPU=8),
+        "C2_cm": dict(describe(cola.Graph.rmat(22, 32, seed=2, reorder="cm"), 8), log_n=22, degree=32, seed=2,
+                      reorder="cm", nGPU=8),
     }
     with open(os.path.join(HERE, "configs.json"), "w") as f:
         json.dump(configs, f, indent=1)

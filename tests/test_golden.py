@@ -1,9 +1,10 @@
 """Golden fixtures (tests/golden/, written by tests/golden/make_fixtures.py).
 
 - The synthetic configurations regenerate bit-identically: the R-MAT
-  generator, symmetrisation, GCN values, Cuthill-McKee order and the row
-  partition are deterministic for a seed whatever the host thread count,
-  so benchmark and profile figures refer to a fixed graph.
+  generator, symmetrisation, GCN values, the community and Cuthill-McKee
+  orders and the row partition are deterministic for a seed whatever the
+  host thread count, so benchmark and profile figures refer to a fixed
+  graph.
 - A small MatrixMarket graph read and serialised by the library gives the
   int32 partition files that the reference's writer (prep_torch_par.py,
   restated with its own scipy routines) produced for it, byte for byte,
@@ -39,8 +40,14 @@ def test_c1_configuration_is_reproduced():
 
 
 def test_c2_configuration_is_reproduced():
-    """The benchmarked 2^22-node graph (about 10 s on 8 host cores)."""
+    """The benchmarked 2^22-node graph in community order (about 30 s on 8
+    host cores)."""
     check_config("C2")
+
+
+def test_c2_cm_configuration_is_reproduced():
+    """The same graph in Cuthill-McKee order (about 10 s)."""
+    check_config("C2_cm")
 
 
 @pytest.mark.parametrize("nGPU", [2, 3])

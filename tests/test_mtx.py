@@ -270,7 +270,7 @@ def bandwidth(g):
     return int(np.max(np.abs(rows - ci))) if len(ci) else 0
 
 
-@pytest.mark.parametrize("method", ["cm", "rcm", "degree"])
+@pytest.mark.parametrize("method", ["cm", "rcm", "degree", "community"])
 def test_reorder_is_a_relabelling(method):
     g = cola.Graph.rmat(10, 6, seed=9)
     h, perm = g.reorder(method)
@@ -295,6 +295,57 @@ def test_cuthill_mckee_recovers_a_band():
     r, perm_r = g.reorder("rcm")
     assert np.array_equal(perm_r, perm[::-1])
     assert bandwidth(r) == bandwidth(h)
+
+
+def cliques(sizes, seed):
+    """Disjoint cliques of the given sizes under shuffled vertex ids."""
+    n = sum(sizes)
+    perm = np.random.default_rng(seed).permutation(n)
+    r, c, groups, o = [], [], [], 0
+    for s in sizes:
+        ids = perm[o:o + s]
+        o += s
+        groups.append(ids)
+        for i in ids:
+            for j in ids:
+                if i != j:
+                    r.append(i)
+                    c.append(j)
+    return cola.Graph.from_coo(n, np.array(r), np.array(c), np.ones(len(r), np.float32)), groups
+
+
+@pytest.mark.parametrize("sizes", [[5, 9, 3, 12, 7, 4], [2] * 30, [40, 3, 3, 3, 20], [6] * 10 + [1] * 5])
+def test_community_order_keeps_cliques_contiguous(sizes):
+    """Disjoint cliques: a vertex first joins a neighbouring singleton, since
+    dQ = 2 (1/2m - d^2/(2m)^2) > 0 whenever d^2 < 2m (true for every clique
+    here), and each later one joins its clique's community with the same
+    sign of gain; no edge joins two cliques, so every clique is one merge
+    tree and gets a contiguous id range (isolated vertices go last)."""
+    g, groups = cliques(sizes, seed=len(sizes))
+    h, perm = g.reorder("community")
+    old2new = np.empty(g.n, np.int64)
+    old2new[perm] = np.arange(g.n)
+    for ids in groups:
+        new = np.sort(old2new[ids])
+        assert new[-1] - new[0] == len(ids) - 1, (len(ids), new)
+    iso = [ids[0] for ids in groups if len(ids) == 1]
+    assert sorted(old2new[iso].tolist()) == list(range(g.n - len(iso), g.n))
+
+
+def test_community_order_is_deterministic_and_balances_rmat():
+    """Same graph, same order; on an R-MAT graph the community order spreads
+    the entries over the row partitions far more evenly than Cuthill-McKee,
+    which puts the hubs' neighbourhood first."""
+    g = cola.Graph.rmat(14, 16, seed=3)
+    _, p1 = g.reorder("community")
+    _, p2 = g.reorder("community")
+    assert np.array_equal(p1, p2)
+    def max_share(h):
+        rp = h.arrays()[0]
+        n = h.n
+        cuts = [rp[min(n, k * n // 8)] for k in range(9)]
+        return max(np.diff(cuts)) / rp[-1]
+    assert max_share(g.reorder("community")[0]) < 0.5 * max_share(g.reorder("cm")[0])
 
 
 def test_reorder_rejects_unknown_method():
@@ -327,7 +378,7 @@ def test_process_graph_and_reorder_mtx_flow(tmp_path):
             assert len(p["csc_colIdx_r"]) <= 2 * 3 * 2
 
 
-@pytest.mark.parametrize("method", ["cm", "rcm", "degree"])
+@pytest.mark.parametrize("method", ["cm", "rcm", "degree", "community"])
 def test_reorder_directed_graph_with_sinks(method):
     """Vertices without out-edges that the breadth-first walk reaches appear
     once in the order (a directed graph, unlike the symmetric R-MAT ones)."""

@@ -115,7 +115,7 @@ def test_mtx_writer_round_trip(tmp_path_factory, g):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("method", ["cm", "rcm", "degree"])
+@pytest.mark.parametrize("method", ["cm", "rcm", "degree", "community"])
 @settings(**dict(SETTINGS, max_examples=30))
 @given(g=graphs(max_n=150))
 def test_reorder_preserves_the_product(method, g):
