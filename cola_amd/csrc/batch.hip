@@ -140,18 +140,7 @@ __global__ __launch_bounds__(256) void batch_hub_reduce_kernel(const HubSeg *__r
   // The table's pointers are read from memory: gld / gst keep the accesses
   // global rather than flat.
   const int c0 = cola_dev::gld(h.hub_ptr + hub), c1 = cola_dev::gld(h.hub_ptr + hub + 1);
-  float s = 0.f, err = 0.f;
-  for (int c = c0; c < c1; ++c) {
-    const float x = cola_dev::gld(h.partials + (int64_t)c * h.pstride + col);
-    if (kahan) {
-      const float y = x - err;
-      const float tt = s + y;
-      err = (tt - s) - y;
-      s = tt;
-    } else {
-      s += x;
-    }
-  }
+  const float s = cola_dev::hub_sum(h.partials + col, h.pstride, c0, c1, kahan != 0);
   float *dst = h.C + (int64_t)cola_dev::gld(h.hub_rows + hub) * h.ldC + col;
   cola_dev::gst(dst, accumulate ? cola_dev::gld(dst) + s : s);
 }

@@ -77,6 +77,36 @@ __device__ __forceinline__ void gst(float4 *p, const float4 &v) {
   *reinterpret_cast<__attribute__((address_space(1))) f32x4_t *>(reinterpret_cast<uint64_t>(p)) = w;
 }
 
+// Sum of a hub row's chunk partials p[c * pstride], c in [c0, c1), added in
+// chunk order (Kahan-compensated when kahan).  32 loads are in flight per
+// round (the last round's loads clamped to the range, its additions
+// predicated), so the additions are exactly the sequential ones while a hub
+// of a few hundred chunks costs about ten memory round trips rather than
+// hundreds.
+__device__ __forceinline__ float hub_sum(const float *p, int64_t pstride, int c0, int c1, bool kahan) {
+  constexpr int kRound = 32;
+  float s = 0.f, err = 0.f;
+  for (int c = c0; c < c1; c += kRound) {
+    float x[kRound];
+#pragma unroll
+    for (int i = 0; i < kRound; ++i) x[i] = gld(p + (int64_t)min(c + i, c1 - 1) * pstride);  // c < c1
+#pragma unroll
+    for (int i = 0; i < kRound; ++i) {
+      if (c + i < c1) {
+        if (kahan) {
+          const float y = x[i] - err;
+          const float t = s + y;
+          err = (t - s) - y;
+          s = t;
+        } else {
+          s += x[i];
+        }
+      }
+    }
+  }
+  return s;
+}
+
 // Vec<DT, EPL>: element type T, the raw register form of one lane's vector,
 // its load (from a pointer, or from a global address: gload) and its
 // widening to FP32.

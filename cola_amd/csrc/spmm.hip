@@ -253,37 +253,7 @@ __global__ __launch_bounds__(256) void hub_reduce_kernel(
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)num_hubs * dim) return;
   const int h = (int)(t / dim), col = (int)(t % dim);
-  const int c0 = hub_ptr[h], c1 = hub_ptr[h + 1];
-  const float *p = partials + col;
-  float s = 0.f, err = 0.f;
-  int c = c0;
-  for (; c + 4 <= c1; c += 4) {
-    float x[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) x[i] = p[(int64_t)(c + i) * pstride];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (kahan) {
-        const float y = x[i] - err;
-        const float tt = s + y;
-        err = (tt - s) - y;
-        s = tt;
-      } else {
-        s += x[i];
-      }
-    }
-  }
-  for (; c < c1; ++c) {
-    const float x = p[(int64_t)c * pstride];
-    if (kahan) {
-      const float y = x - err;
-      const float tt = s + y;
-      err = (tt - s) - y;
-      s = tt;
-    } else {
-      s += x;
-    }
-  }
+  const float s = cola_dev::hub_sum(partials + col, pstride, hub_ptr[h], hub_ptr[h + 1], kahan != 0);
   float *dst = C + (int64_t)hub_rows[h] * ldC + col;
   *dst = accumulate ? *dst + s : s;
 }
