@@ -372,6 +372,18 @@ def test_hot_cold_plans_are_not_batched(c1):
         cola.Plan(c1.part, hot_cols=100, col_panels=2)
 
 
+def test_tuning_ranges(c1):
+    """col_tile -k means k x 128 bytes for k = 1..8; anything further out is
+    rejected with a status, as are out-of-range knobs."""
+    plan = cola.Plan(c1.part)
+    plan.tuning = dict(col_tile=-8)
+    assert plan.tuning["col_tile"] == -8
+    for bad in (dict(col_tile=-9), dict(rpw_long=65), dict(unroll=12), dict(xcd_remap=2)):
+        with pytest.raises(cola.ColaError) as e:
+            plan.tuning = bad
+        assert e.value.code == cola._lib.COLA_ERR_INVALID, bad
+
+
 def test_linearity_in_B(c1):
     """A·(B1 + B2) = A·B1 + A·B2 to FP32 rounding."""
     plan = cola.Plan(c1.part)
@@ -514,8 +526,10 @@ TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=
            # 16 loads in flight on groups narrower than 16 lanes: each lane
            # stages 2, 4 or 8 entries per chunk
            dict(col_tile=32, rpw_short=8, rpw_long=8, blocks_per_cu=2, xcd_remap=0, split_remote=0, unroll=16),
-           # the large-graph default: 128-byte tiles, 1 block per CU
+           # 128-byte tiles, 1 block per CU (the Cuthill-McKee partition's best)
            dict(col_tile=-1, rpw_short=0, rpw_long=8, blocks_per_cu=1, xcd_remap=0, split_remote=0, unroll=16),
+           # the large-graph default: 256-byte tiles, XCD remap
+           dict(col_tile=-2, rpw_short=0, rpw_long=4, blocks_per_cu=2, xcd_remap=1, split_remote=0, unroll=8),
            dict(col_tile=16, rpw_short=16, rpw_long=8, blocks_per_cu=3, xcd_remap=1, split_remote=1, unroll=16),
            dict(col_tile=8, rpw_short=32, rpw_long=4, blocks_per_cu=2, xcd_remap=0, split_remote=0, unroll=16)]
 
