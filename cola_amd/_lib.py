@@ -102,6 +102,7 @@ SIGNATURES = {
     "cola_partition_get_arrays": [c_vp, ctypes.POINTER(PartitionArrays)],
     "cola_partition_destroy": [c_vp],
     "cola_serialize": [c_vp, c_char_p, c_char_p, c_int, c_int],
+    "cola_serialize_ppl": [c_vp, c_char_p, c_char_p, c_int, c_int],
     "cola_partition_load": [c_char_p, c_char_p, c_int, c_int, ctypes.POINTER(c_vp)],
     "cola_nnz_split": [c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, P_i64, P_i32],
     "cola_zip_create": [c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, ctypes.POINTER(c_vp)],

@@ -960,6 +960,37 @@ cola_status cola_serialize(cola_graph g, const char *bin_path,
   return COLA_OK;
 }
 
+cola_status cola_serialize_ppl(cola_graph g, const char *bin_path, const char *graph_name, int nGPU,
+                               int group_size) {
+  if (!g || !bin_path || !graph_name) return fail(COLA_ERR_INVALID, "serialize_ppl: NULL argument");
+  if (nGPU < 2 || group_size < 1 || group_size >= nGPU)
+    return fail(COLA_ERR_INVALID, "serialize_ppl: need 1 <= group_size < nGPU");
+  const Graph &G = g->g;
+  const int64_t n = G.n;
+  const int stages = nGPU / group_size;
+  const int64_t P = (n + nGPU - 1) / nGPU;
+  const std::string dir = std::string(bin_path) + "/" + graph_name + "/ppl" + std::to_string(stages);
+  cola_status st = mkdirs(dir);
+  if (st) return st;
+  // Stage s covers rows [s*gs*P, min((s+1)*gs*P, n)); its file holds the
+  // sorted unique columns of those rows.
+  std::vector<uint8_t> seen(n, 0);
+  std::vector<int32_t> cols;
+  for (int s = 0; s < stages; ++s) {
+    const int64_t lb = std::min<int64_t>(n, (int64_t)s * group_size * P);
+    const int64_t ub = std::min<int64_t>(n, (int64_t)(s + 1) * group_size * P);
+    std::fill(seen.begin(), seen.end(), 0);
+    for (int64_t k = G.row_ptr[lb]; k < G.row_ptr[ub]; ++k) seen[G.col_idx[k]] = 1;
+    cols.clear();
+    for (int64_t c = 0; c < n; ++c)
+      if (seen[c]) cols.push_back((int32_t)c);
+    const std::string path = dir + "/colIdx_" + std::to_string(nGPU) + "_" + std::to_string(stages) + "_" +
+                             std::to_string(s) + ".bin";
+    if ((st = vec2bin(path, cols.data(), cols.size()))) return st;
+  }
+  return COLA_OK;
+}
+
 cola_status cola_partition_load(const char *bin_path, const char *graph_name,
                                 int nGPU, int rank, cola_partition *out) {
   if (!bin_path || !graph_name || !out) return fail(COLA_ERR_INVALID, "partition_load: NULL argument");

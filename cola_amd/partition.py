@@ -6,6 +6,8 @@ Reference behaviour mirrored (paths relative to the CoLa tree):
   prep_torch_par.py:24-79   row_partition  -> Partition.row_partition / Graph.partition
   rabbit_order_write.py:46-77  reorder + mmwrite -> Graph.reorder / Graph.write_mtx
   prep_torch_par.py:93-110  serialize      -> serialize
+  prep_torch_ppl.py:24-84   row_partition / serialize / process_graph (stage files)
+                            -> serialize_ppl / process_graph_ppl, stage_cols
   src/distSpMM_ppl_batch.cu:49-99  partition load + 1.0 values -> Partition.load
   include/dist_spmm_ppl_batch.cuh:7-27   bin2vec   -> bin2vec
   include/dist_spmm_ppl_batch.cuh:29-56  nnz_split -> nnz_split
@@ -236,6 +238,32 @@ def serialize(graph, bin_path, graph_name, nGPU_list, write_values=True):
     for nGPU in nGPU_list:
         check(lib().cola_serialize(graph.handle, os.fsencode(bin_path), graph_name.encode(), int(nGPU),
                                    1 if write_values else 0))
+
+
+def serialize_ppl(graph, bin_path, graph_name, nGPU_list, gs_list):
+    """The stage files of the pipelined path (prep_torch_ppl.py:53-76
+    serialize): for every nGPU and group size gs < nGPU, the sorted unique
+    columns of each stage's rows under
+    <bin_path>/<graph_name>/ppl<nGPU // gs>/colIdx_<nGPU>_<nGPU // gs>_<s>.bin."""
+    for nGPU in nGPU_list:
+        for gs in gs_list:
+            if gs < nGPU:
+                check(lib().cola_serialize_ppl(graph.handle, os.fsencode(bin_path), graph_name.encode(), int(nGPU),
+                                               int(gs)))
+
+
+def stage_cols(bin_path, graph_name, nGPU, gs, stage):
+    """One stage file written by serialize_ppl (the array
+    distSpMM_ppl_batch.cu:102-104 reads as colIdx_workgroup)."""
+    S = nGPU // gs
+    return bin2vec(os.path.join(bin_path, graph_name, f"ppl{S}", f"colIdx_{nGPU}_{S}_{stage}.bin"))
+
+
+def process_graph_ppl(graph_name, mtx_path, bin_path, nGPU_list, gs_list):
+    """load_mtx + serialize_ppl (prep_torch_ppl.py:79-84).  Returns the Graph."""
+    g = load_mtx(graph_name, mtx_path)
+    serialize_ppl(g, bin_path, graph_name, nGPU_list, gs_list)
+    return g
 
 
 def load_mtx(graph_name, mtx_path, values="one"):

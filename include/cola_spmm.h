@@ -188,6 +188,16 @@ cola_status cola_partition_destroy(cola_partition p);
 cola_status cola_serialize(cola_graph g, const char *bin_path,
                            const char *graph_name, int nGPU, int write_values);
 
+/* prep_torch_ppl.py:24-49 row_partition: the stage files of the pipelined
+ * multi-stage path.  With S = nGPU / group_size stages and P =
+ * ceil(n/nGPU), stage s covers rows [s*group_size*P, min((s+1)*group_size*P,
+ * n)); <bin_path>/<graph_name>/ppl<S>/colIdx_<nGPU>_<S>_<s>.bin holds the
+ * sorted unique columns of those rows (int32), the files
+ * distSpMM_ppl_batch.cu:102-104 reads.  Needs 1 <= group_size < nGPU (the
+ * reference writes nothing otherwise, prep_torch_ppl.py:64). */
+cola_status cola_serialize_ppl(cola_graph g, const char *bin_path, const char *graph_name, int nGPU,
+                               int group_size);
+
 /* distSpMM_ppl_batch.cu:49-99 partition load: reads global_rowPtr.bin (for n)
  * and the six per-rank files with size and range checks (bin2vec,
  * dist_spmm_ppl_batch.cuh:7-27, without its 2 GiB int limit); value files are

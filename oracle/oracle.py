@@ -18,6 +18,12 @@ Contents:
     the global files and every rank's six partition files as int32 arrays.
     Pinned by the structural invariants of row_partition_np and by equality
     with row_partition_np on the deduplicated CSR (tests/test_mtx.py).
+  - ppl_files_from_mtx: prep_torch_ppl.py:24-49 for a MatrixMarket file,
+    restated with the routines it calls (scipy.io.mmread, np.unique): each
+    stage's sorted unique columns.  Pinned by brute force: every stage's
+    columns are exactly the columns of the stage's rows in the dense
+    matrix, and the union over stages covers the rows the stages span
+    (tests/test_mtx.py).
   - dense_partition_product: brute-force dense A_p·B for tiny graphs.
 """
 import ctypes
@@ -231,6 +237,27 @@ def par_files_from_mtx(path, nGPU):
         out[("csc_colIdx_r", rank)] = np.unique(np.asarray(rc, dtype=np.int32))
         out[("csc_colPtr_r", rank)] = np.unique(np.asarray(R.indptr, dtype=np.int32))
         out[("csc_rowIdx_r", rank)] = np.asarray(R.indices, dtype=np.int32)
+    return out
+
+
+def ppl_files_from_mtx(path, nGPU, gs):
+    """prep_torch_ppl.py:24-49 row_partition for one MatrixMarket file, one
+    GPU count and one group size: {stage: sorted unique columns of the
+    stage's rows} with the dtype the reference writes (mmread's COO column
+    dtype, not cast: int32 here)."""
+    import scipy.io as sio
+    A = sio.mmread(path)                                   # ppl.py:18
+    row, col = np.asarray(A.row), np.asarray(A.col)        # ppl.py:30-31
+    num_nodes = A.shape[0]                                 # ppl.py:32
+    stages = nGPU // gs                                    # ppl.py:26
+    P = (num_nodes + nGPU - 1) // nGPU                     # ppl.py:34
+    out = {}
+    for s in range(stages):                                # ppl.py:37-39
+        lb, ub = s * gs * P, min((s + 1) * gs * P, num_nodes)
+        m = (row >= lb) & (row < ub)                       # ppl.py:45
+        u = np.unique(col[m])                              # ppl.py:47
+        assert np.all(np.diff(u) > 0)                      # ppl.py:49
+        out[s] = u
     return out
 
 

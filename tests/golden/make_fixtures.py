@@ -11,6 +11,9 @@ root: python tests/golden/make_fixtures.py).
   hub row, a symmetric half) and the int32 partition files the reference's
   writer produces for it (prep_torch_par.py:14-101, restated with the same
   scipy routines in oracle.par_files_from_mtx) for 2 and 3 GPUs.
+- ppl_small.npz: the stage files prep_torch_ppl.py:24-49 writes for the same
+  graph (oracle.ppl_files_from_mtx) for (nGPU, group size) = (4, 1), (4, 2),
+  (8, 2).
 """
 import json
 import os
@@ -65,7 +68,12 @@ def main():
             name = k if isinstance(k, str) else f"{k[0]}_{nGPU}_{k[1]}"
             arrays[name] = v
     np.savez(os.path.join(HERE, "par_small.npz"), **arrays)
-    print("wrote configs.json, par_small.mtx, par_small.npz")
+    stages = {}
+    for nGPU, gs in ((4, 1), (4, 2), (8, 2)):
+        for s, cols in orc.ppl_files_from_mtx(path, nGPU, gs).items():
+            stages[f"colIdx_{nGPU}_{gs}_{s}"] = cols
+    np.savez(os.path.join(HERE, "ppl_small.npz"), **stages)
+    print("wrote configs.json, par_small.mtx, par_small.npz, ppl_small.npz")
 
 
 if __name__ == "__main__":

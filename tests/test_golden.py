@@ -8,7 +8,7 @@
 - A small MatrixMarket graph read and serialised by the library gives the
   int32 partition files that the reference's writer (prep_torch_par.py,
   restated with its own scipy routines) produced for it, byte for byte,
-  without scipy at test time.
+  without scipy at test time, and the stage files of prep_torch_ppl.py.
 """
 import json
 import os
@@ -48,6 +48,18 @@ def test_c2_configuration_is_reproduced():
 def test_c2_cm_configuration_is_reproduced():
     """The same graph in Cuthill-McKee order (about 10 s)."""
     check_config("C2_cm")
+
+
+@pytest.mark.parametrize("nGPU,gs", [(4, 1), (4, 2), (8, 2)])
+def test_stage_files_of_an_mtx_graph(tmp_path, nGPU, gs):
+    """The stage files (prep_torch_ppl.py:24-49) of the same small graph."""
+    ref = np.load(os.path.join(HERE, "ppl_small.npz"))
+    g = cola.load_mtx("par_small", HERE)
+    cola.serialize_ppl(g, str(tmp_path), "par_small", [nGPU], [gs])
+    S = nGPU // gs
+    for s in range(S):
+        got = (tmp_path / "par_small" / f"ppl{S}" / f"colIdx_{nGPU}_{S}_{s}.bin").read_bytes()
+        assert got == ref[f"colIdx_{nGPU}_{gs}_{s}"].tobytes(), s
 
 
 @pytest.mark.parametrize("nGPU", [2, 3])

@@ -393,3 +393,80 @@ def test_reorder_directed_graph_with_sinks(method):
     P.sort_indices()
     rp2, ci2, va2 = h.arrays()
     assert np.array_equal(rp2, P.indptr) and np.array_equal(ci2, P.indices) and np.array_equal(va2, P.data)
+
+
+# ---------------------------------------------------------------------------
+# Stage files of the pipelined path (prep_torch_ppl.py:24-84).
+# ---------------------------------------------------------------------------
+PPL_MTX = """%%MatrixMarket matrix coordinate real symmetric
+% a hub row, duplicates, an explicit zero, isolated vertices at the end
+13 13 16
+1 1 1.0
+2 1 0.5
+5 2 1.0
+5 2 2.0
+7 3 0.0
+4 4 1.0
+9 1 1.0
+9 2 1.0
+9 3 1.0
+9 4 1.0
+9 5 1.0
+9 6 1.0
+9 7 1.0
+9 8 1.0
+10 6 1.0
+11 10 1.0
+"""
+
+
+def structure(path):
+    A = sio.mmread(path)
+    S = np.zeros(A.shape, bool)
+    S[np.asarray(A.row), np.asarray(A.col)] = True
+    return S
+
+
+@pytest.mark.parametrize("nGPU,gs", [(2, 1), (4, 1), (4, 2), (8, 2), (8, 3), (16, 1), (16, 8)])
+def test_ppl_oracle_against_brute_force(tmp_path, nGPU, gs):
+    """Each stage's columns are exactly the columns with an entry (explicit
+    zeros included, as in the COO the reference reads) in the stage's rows."""
+    path = write(tmp_path / "p.mtx", PPL_MTX)
+    S = structure(path)
+    n = S.shape[0]
+    P = (n + nGPU - 1) // nGPU
+    ref = orc.ppl_files_from_mtx(path, nGPU, gs)
+    assert sorted(ref) == list(range(nGPU // gs))
+    for s, cols in ref.items():
+        lb, ub = min(n, s * gs * P), min(n, (s + 1) * gs * P)
+        assert np.array_equal(cols, np.flatnonzero(S[lb:ub].any(axis=0))), s
+
+
+@pytest.mark.parametrize("nGPU,gs", [(2, 1), (4, 1), (4, 2), (8, 2), (8, 3), (16, 1), (16, 8)])
+def test_stage_files_match_the_reference_writer(tmp_path, nGPU, gs):
+    """serialize_ppl on the library's reading of an .mtx file writes, byte for
+    byte, what prep_torch_ppl.py writes (restated in oracle.ppl_files_from_mtx),
+    including empty stages past the last row."""
+    mtx = tmp_path / "mtx"
+    mtx.mkdir()
+    path = write(mtx / "p.mtx", PPL_MTX)
+    sp_dir = tmp_path / "SP"
+    cola.process_graph_ppl("p", str(mtx), str(sp_dir), [nGPU], [gs])
+    S = nGPU // gs
+    for s, cols in orc.ppl_files_from_mtx(path, nGPU, gs).items():
+        f = sp_dir / "p" / f"ppl{S}" / f"colIdx_{nGPU}_{S}_{s}.bin"
+        assert f.read_bytes() == cols.astype(np.int32).tobytes(), s
+        assert np.array_equal(cola.stage_cols(str(sp_dir), "p", nGPU, gs, s), cols)
+
+
+def test_stage_files_follow_the_reference_loop(tmp_path):
+    """serialize_ppl writes only group sizes below nGPU (prep_torch_ppl.py:64);
+    the C entry point rejects the others with a status."""
+    g = cola.Graph.rmat(8, 4, seed=3)
+    cola.serialize_ppl(g, str(tmp_path), "g", [2, 4], [1, 4, 8])
+    names = sorted(p.relative_to(tmp_path).as_posix() for p in tmp_path.rglob("*.bin"))
+    assert names == ["g/ppl2/colIdx_2_2_0.bin", "g/ppl2/colIdx_2_2_1.bin", "g/ppl4/colIdx_4_4_0.bin",
+                     "g/ppl4/colIdx_4_4_1.bin", "g/ppl4/colIdx_4_4_2.bin", "g/ppl4/colIdx_4_4_3.bin"]
+    for nGPU, gs in ((4, 4), (4, 0), (1, 1)):
+        with pytest.raises(ColaError):
+            _lib.check(_lib.lib().cola_serialize_ppl(g.handle, str(tmp_path).encode(), b"g", nGPU, gs))
