@@ -17,8 +17,9 @@ Extras (same partition unless stated): reduced-precision storage of B
 (BF16 / FP16 / FP8-e4m3 + row scales) at dims 64 / 128 / 256 with relative
 Frobenius and max-row error against the FP64 oracle (computed on the GPU in
 FP64 over sampled rows); the reference-semantics path (nnz_split + CSC
-scatter atomics); the batched form (64 SpMMs, one persistent launch); the
-served form (Poisson arrivals over 200 resident graphs).
+scatter atomics); every row partition of the graph timed one after another
+(FP32, default schedule); the batched form (64 SpMMs, one persistent launch);
+the served form (Poisson arrivals over 200 resident graphs).
 """
 import argparse
 import json
@@ -118,6 +119,38 @@ def precision_section(cola, torch, orc, log, ctx, args, f32_tuning):
             for k, v in per.items()))
         del Bf, sample
     plan.tuning = f32_tuning
+    return out
+
+
+def partitions_section(cola, torch, log, g, B, args):
+    """Every row partition of the benchmarked graph, one after another on this
+    GPU: pack + cola_spmm_partition in FP32 at the plan's default schedule,
+    so the shares of a sharded step can be compared (each figure is one
+    partition on one GPU)."""
+    out = []
+    P = (g.n + args.nparts - 1) // args.nparts
+    N = B.shape[1]
+    for r in range(args.nparts):
+        part = g.partition(args.nparts, r)
+        info = part.info()
+        rec = {"rank": r, "nnz": int(info["nnz_l"] + info["nnz_r"]), "unique_remote_rows": int(info["U_r"])}
+        if rec["nnz"] == 0:
+            rec["ms_per_step"] = 0.0
+            out.append(rec)
+            continue
+        plan = cola.Plan(part, row_order=args.row_order)
+        packed = torch.empty((max(plan.U_r, 1), N), dtype=torch.float32, device=B.device)
+        C = torch.empty((P, N), dtype=torch.float32, device=B.device)
+        B_local = B[r * P:(r + 1) * P]
+
+        def step():
+            plan.pack(B, packed)
+            plan.spmm(C, B_local, packed, flags=cola.OVERWRITE)
+
+        rec["ms_per_step"] = timed(torch, step, max(3, args.steps // 4), warm=2)
+        out.append(rec)
+        del plan, packed, C
+    log("partitions: " + ", ".join(f"{x['rank']}: {x['nnz']} entries {x['ms_per_step']:.3f} ms" for x in out))
     return out
 
 
@@ -274,6 +307,7 @@ def main():
     ap.add_argument("--precision-dims", type=int, nargs="*", default=[64, 128, 256])
     ap.add_argument("--no-reference-path", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-SpMM persistent-launch section")
+    ap.add_argument("--no-partitions", action="store_true", help="skip timing every partition of the graph")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
     ap.add_argument("--serve-graphs", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
@@ -458,6 +492,8 @@ def main():
                 "path": "pack + cola_ppl0_batch_SpMM: nnz_split virtual rows, FP32 atomics"}
             log(f"reference API wrapper {ms_ref:.3f} ms; literal ppl0 restatement (atomics) {ms_lit:.3f} ms")
             del Cr, zl, zr, uniq, packed32
+        if not args.no_partitions and args.dtype == "f32":
+            extras["partitions"] = partitions_section(cola, torch, log, g, B, args)
         dim_tunings = {}
         if not args.no_batched:
             extras["batched"], dim_tunings = batched_section(cola, torch, log)
