@@ -59,7 +59,7 @@ class PlanOpts(ctypes.Structure):
 class PlanTuning(ctypes.Structure):
     _fields_ = [("col_tile", c_i32), ("rpw_short", c_i32), ("rpw_long", c_i32), ("blocks_per_cu", c_i32),
                 ("xcd_remap", c_i32), ("split_remote", c_i32),
-                ("unroll", c_i32), ("panels", c_i32)]
+                ("unroll", c_i32), ("panels", c_i32), ("blocks_per_cu_short", c_i32)]
 
 
 class BatchProblem(ctypes.Structure):

@@ -556,7 +556,8 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
         a.xcd_remap = t.xcd_remap ? 1 : 0;
         a.phase = phase;
         const int64_t waves = (bin->n + a.rpw - 1) / a.rpw;
-        int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bpc);
+        const int bin_bpc = bi == 1 && t.blocks_per_cu_short > 0 ? t.blocks_per_cu_short : bpc;
+        int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bin_bpc);
         blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
         if (!launch_rows(c.dtype, sh, a, (int)blocks, s, t.unroll == 16))
           return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: no kernel for this dim");
@@ -804,8 +805,8 @@ static cola_status plan_from_partition(const cola::Partition &part, const cola_p
   // than the L2s hold and want 256-byte column tiles, 4 rows per wave at 2
   // blocks per CU and each XCD on its own item range; small ones want every
   // column in one pass and many waves.
-  p->tune = hp.nnz >= (int64_t(1) << 22) ? cola_plan_tuning{-2, 0, 4, 2, 1, 0, 8, 0}
-                                         : cola_plan_tuning{0, 0, 1, 8, 1, 0, 8, 0};
+  p->tune = hp.nnz >= (int64_t(1) << 22) ? cola_plan_tuning{-2, 0, 4, 2, 1, 0, 8, 0, 4}
+                                         : cola_plan_tuning{0, 0, 1, 8, 1, 0, 8, 0, 0};
   p->num_empty = (int64_t)hp.empty_rows.size();
   p->num_hubs = (int64_t)hp.hub_rows.size();
   p->num_chunks = hp.hub_chunk_ptr.back();
@@ -972,7 +973,8 @@ cola_status cola_plan_get_info(cola_plan p, cola_plan_info *info) {
 cola_status cola_plan_set_tuning(cola_plan p, const cola_plan_tuning *t) {
   if (!p || !t) return fail(COLA_ERR_INVALID, "plan_set_tuning: NULL argument");
   if (t->col_tile < -8 || t->rpw_short < 0 || t->rpw_short > 64 || t->rpw_long < 0 || t->rpw_long > 64 ||
-      t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || (t->xcd_remap != 0 && t->xcd_remap != 1) ||
+      t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || t->blocks_per_cu_short < 0 || t->blocks_per_cu_short > 64 ||
+      (t->xcd_remap != 0 && t->xcd_remap != 1) ||
       (t->split_remote != 0 && t->split_remote != 1) || (t->unroll != 0 && t->unroll != 8 && t->unroll != 16) ||
       (t->panels != 0 && t->panels != 1))
     return fail(COLA_ERR_INVALID, "plan_set_tuning: parameter out of range");
@@ -1054,10 +1056,10 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
     seen.emplace_back(ms, t);
     if (verbose)
       fprintf(stderr,
-              "[autotune %s] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d split=%d unroll=%d "
+              "[autotune %s] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d/%d xcd=%d split=%d unroll=%d "
               "panels=%d: %.4f ms\n",
-              tag, t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.xcd_remap, t.split_remote, t.unroll,
-              t.panels, ms);
+              tag, t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.blocks_per_cu_short, t.xcd_remap,
+              t.split_remote, t.unroll, t.panels, ms);
     if (seen.size() == 1 || ms < bms * 0.99f) {
       bms = ms;
       best = t;
@@ -1108,6 +1110,11 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
       t.blocks_per_cu = b;
       cands.push_back(t);
     }
+    for (int b : {0, 2, 4, 8, 16}) {
+      cola_plan_tuning t = best;
+      t.blocks_per_cu_short = b;
+      cands.push_back(t);
+    }
     for (int x : {0, 1}) {
       cola_plan_tuning t = best;
       t.xcd_remap = x;
@@ -1154,10 +1161,11 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
       if (verbose)
         for (size_t i = 0; i < k; ++i)
           fprintf(stderr,
-                  "[autotune final] #%zu tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d xcd=%d unroll=%d: "
+                  "[autotune final] #%zu tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d/%d xcd=%d unroll=%d: "
                   "%.4f ms%s\n",
                   i, seen[i].second.col_tile, seen[i].second.rpw_short, seen[i].second.rpw_long,
-                  seen[i].second.blocks_per_cu, seen[i].second.xcd_remap, seen[i].second.unroll, score[i],
+                  seen[i].second.blocks_per_cu, seen[i].second.blocks_per_cu_short, seen[i].second.xcd_remap,
+                  seen[i].second.unroll, score[i],
                   i == w ? " <- best" : "");
     }
   }

@@ -336,11 +336,11 @@ cola_status cola_plan_get_info(cola_plan plan, cola_plan_info *info);
 
 /* Launch parameters of the plan's kernels (the autotuner's search space).
  * A new plan with at least 2^22 entries starts with {-2, 0, 4, 2, 1, 0, 8,
- * 0}: 256-byte column tiles, 4 rows per wave on long rows, 2 blocks per CU,
- * XCD remap, 8 loads in flight (the tuned schedule of a partition of the
- * community-ordered 2^22-node R-MAT graph); a
- * smaller one with {0, 0, 1, 8, 1, 0, 8, 0}: all columns in one pass, one
- * row per wave on long rows, 8 blocks per CU. */
+ * 0, 4}: 256-byte column tiles, 4 rows per wave on long rows, 2 blocks per
+ * CU (4 for the short-row bin), XCD remap, 8 loads in flight (the tuned
+ * schedule of a partition of the community-ordered 2^22-node R-MAT graph);
+ * a smaller one with {0, 0, 1, 8, 1, 0, 8, 0, 0}: all columns in one pass,
+ * one row per wave on long rows, 8 blocks per CU. */
 typedef struct {
   int32_t col_tile;      /* dense columns per pass over B (0 = all of dim;
                             -k, k = 1..8: k x 128 bytes of the storage
@@ -360,6 +360,10 @@ typedef struct {
   int32_t panels;        /* 1: run the column-panel schedule (plans built
                             with col_panels > 1): C is zeroed (overwrite),
                             then one accumulating pass per panel; 0: off    */
+  int32_t blocks_per_cu_short; /* grid of the short-row bin's launches
+                            (0 = blocks_per_cu): short rows are chains of
+                            dependent loads and want more waves in flight
+                            than long rows                                  */
 } cola_plan_tuning;
 cola_status cola_plan_set_tuning(cola_plan plan, const cola_plan_tuning *t);
 cola_status cola_plan_get_tuning(cola_plan plan, cola_plan_tuning *t);

@@ -378,7 +378,7 @@ def test_tuning_ranges(c1):
     plan = cola.Plan(c1.part)
     plan.tuning = dict(col_tile=-8)
     assert plan.tuning["col_tile"] == -8
-    for bad in (dict(col_tile=-9), dict(rpw_long=65), dict(unroll=12), dict(xcd_remap=2)):
+    for bad in (dict(col_tile=-9), dict(rpw_long=65), dict(unroll=12), dict(xcd_remap=2), dict(blocks_per_cu_short=65)):
         with pytest.raises(cola.ColaError) as e:
             plan.tuning = bad
         assert e.value.code == cola._lib.COLA_ERR_INVALID, bad
@@ -528,8 +528,11 @@ TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=
            dict(col_tile=32, rpw_short=8, rpw_long=8, blocks_per_cu=2, xcd_remap=0, split_remote=0, unroll=16),
            # 128-byte tiles, 1 block per CU (the Cuthill-McKee partition's best)
            dict(col_tile=-1, rpw_short=0, rpw_long=8, blocks_per_cu=1, xcd_remap=0, split_remote=0, unroll=16),
-           # the large-graph default: 256-byte tiles, XCD remap
-           dict(col_tile=-2, rpw_short=0, rpw_long=4, blocks_per_cu=2, xcd_remap=1, split_remote=0, unroll=8),
+           # the large-graph default: 256-byte tiles, XCD remap, short rows on a larger grid
+           dict(col_tile=-2, rpw_short=0, rpw_long=4, blocks_per_cu=2, xcd_remap=1, split_remote=0, unroll=8,
+                blocks_per_cu_short=4),
+           dict(col_tile=0, rpw_short=2, rpw_long=1, blocks_per_cu=1, xcd_remap=0, split_remote=1, unroll=16,
+                blocks_per_cu_short=16),
            dict(col_tile=16, rpw_short=16, rpw_long=8, blocks_per_cu=3, xcd_remap=1, split_remote=1, unroll=16),
            dict(col_tile=8, rpw_short=32, rpw_long=4, blocks_per_cu=2, xcd_remap=0, split_remote=0, unroll=16)]
 

@@ -63,7 +63,7 @@ tunings = st.fixed_dictionaries(dict(
     col_tile=st.sampled_from([0, 8, 16, 32, 64]), rpw_short=st.sampled_from([0, 1, 2, 4, 8, 16, 32, 64]),
     rpw_long=st.sampled_from([0, 1, 2, 4, 8]), blocks_per_cu=st.sampled_from([1, 2, 4, 8]),
     xcd_remap=st.sampled_from([0, 1]), split_remote=st.sampled_from([0, 1]), unroll=st.sampled_from([8, 16]),
-    panels=st.sampled_from([0, 1])))
+    panels=st.sampled_from([0, 1]), blocks_per_cu_short=st.sampled_from([0, 0, 4, 16])))
 
 
 @settings(max_examples=150, **SETTINGS)
