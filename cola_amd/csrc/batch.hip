@@ -109,6 +109,7 @@ __global__ __launch_bounds__(256) void batch_rows_kernel(const BatchSeg *__restr
     a.rpw = sg.rpw;
     a.xcd_remap = 0;
     a.phase = 0;
+    a.abase = 0;  // segments keep full 64-bit row addresses
     const int it0 = grp * sg.rpw;
     // The lane-group width is wave-uniform (one segment per task): each case
     // is the plan kernel's step with that width compiled in.

@@ -218,6 +218,10 @@ struct RowArgs {
   int phase;      // 0: all entries; 1: local entries only; 2: remote entries
                   // only, added to what phase 1 wrote; 3: all entries, rows
                   // added to what the first pass of a hot/cold plan wrote
+  uint64_t abase; // nonzero: every row address of Bl and Bp is abase + a
+                  // 32-bit multiple of 16 bytes (the host checked the span
+                  // and alignment), so one word per entry is broadcast
+                  // instead of two; 0: full 64-bit addresses
 };
 
 // Two forms of the staging (PIPE): false turns the next chunk's staged
@@ -299,8 +303,13 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
       const bool loc = c >= 0;
       const T *rp = loc ? Bl + (int64_t)c * a.ldB : Bp + (int64_t)(~c) * a.ldP;
       const uint64_t p = reinterpret_cast<uint64_t>(rp);
-      mlo_next[k] = (uint32_t)p;
-      mhi_next[k] = (uint32_t)(p >> 32);
+      if (a.abase) {
+        mlo_next[k] = (uint32_t)((p - a.abase) >> 4);
+        mhi_next[k] = 0;
+      } else {
+        mlo_next[k] = (uint32_t)p;
+        mhi_next[k] = (uint32_t)(p >> 32);
+      }
       mv_next[k] = base + k * tsize + tl < e ? nx[k] : 0.f;
       if constexpr (DT == COLA_FP8_E4M3) msc_next[k] = gld(loc ? a.sl + c : a.sp + ~c);
     }
@@ -324,10 +333,12 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
   // This lane's offset into a B row, per vector (clamped to the row start
   // for the lanes past dim, whose results are never stored).
   int lcol[NV];
+  uint64_t lbase[NV];  // with abase: this lane's column in the common table
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     const int col = (k * G + gl) * EPL;
     lcol[k] = col < a.dim ? col : 0;
+    lbase[k] = a.abase + (uint64_t)lcol[k] * sizeof(T);
   }
   if (b < e) {
     stage_load(b);
@@ -344,22 +355,38 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
       // Broadcast the batch's (row address, value) pairs first, then issue
       // every load.  With K > 1 the batch is the whole chunk (j = 0) and
       // slot u reads entry u / G of team lane (u % G) * gpt + tsub.
-      uint64_t pu[U];
       float av[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int k = K > 1 ? u / G : 0;
-        const int src = (tbase + (K > 1 ? (u % G) * gpt + tsub : j + u * gpt + tsub)) << 2;
-        const uint32_t lo = __builtin_amdgcn_ds_bpermute(src, (int)mlo[k]);
-        const uint32_t hi = __builtin_amdgcn_ds_bpermute(src, (int)mhi[k]);
-        pu[u] = ((uint64_t)hi << 32) | lo;
-        av[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv[k])));
-      }
       Raw raw[U][NV];
+      if (a.abase) {
+        // One word per entry: the row's 16-byte offset from abase.
+        uint32_t ou[U];
 #pragma unroll
-      for (int u = 0; u < U; ++u)
+        for (int u = 0; u < U; ++u) {
+          const int k = K > 1 ? u / G : 0;
+          const int src = (tbase + (K > 1 ? (u % G) * gpt + tsub : j + u * gpt + tsub)) << 2;
+          ou[u] = __builtin_amdgcn_ds_bpermute(src, (int)mlo[k]);
+          av[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv[k])));
+        }
 #pragma unroll
-        for (int k = 0; k < NV; ++k) raw[u][k] = V::gload(pu[u] + (uint64_t)lcol[k] * sizeof(T));
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int k = 0; k < NV; ++k) raw[u][k] = V::gload(lbase[k] + ((uint64_t)ou[u] << 4));
+      } else {
+        uint64_t pu[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int k = K > 1 ? u / G : 0;
+          const int src = (tbase + (K > 1 ? (u % G) * gpt + tsub : j + u * gpt + tsub)) << 2;
+          const uint32_t lo = __builtin_amdgcn_ds_bpermute(src, (int)mlo[k]);
+          const uint32_t hi = __builtin_amdgcn_ds_bpermute(src, (int)mhi[k]);
+          pu[u] = ((uint64_t)hi << 32) | lo;
+          av[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv[k])));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int k = 0; k < NV; ++k) raw[u][k] = V::gload(pu[u] + (uint64_t)lcol[k] * sizeof(T));
+      }
       if constexpr (PIPE) {
         if (more && j == 0) {
           // Pin the staged registers here, after this batch's row loads (the

@@ -452,6 +452,25 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
   }
   const int accumulate = (c.flags & COLA_OVERWRITE) ? 0 : 1;
   const int kahan = (c.flags & COLA_KAHAN) ? 1 : 0;
+  // Common base of the two row tables of a column tile starting at column
+  // c0 (w columns): nonzero when every row an entry can address (local rows
+  // below nodesPerPE, packed rows below U_r) lies within 2^36 bytes of it
+  // and starts on a 16-byte boundary, so the row kernel can broadcast a
+  // 32-bit offset in 16-byte units instead of a 64-bit address.
+  auto table_base = [&](int c0, int w) -> uint64_t {
+    if (!vec_ok) return 0;
+    uint64_t lo = UINT64_MAX, hi = 0;
+    auto span = [&](const void *base, int64_t rows, int64_t ld) {
+      if (!base || rows <= 0) return;
+      const uint64_t b = (uint64_t)(uintptr_t)base + (uint64_t)c0 * eb;
+      lo = std::min(lo, b);
+      hi = std::max(hi, b + ((uint64_t)(rows - 1) * (uint64_t)ld + (uint64_t)w) * eb);
+    };
+    span(c.Bl, p->nodesPerPE, c.ldB);
+    span(c.Bp, p->U_r, c.ldP);
+    if (lo == UINT64_MAX || lo % 16 != 0 || hi - lo >= (uint64_t(1) << 36)) return 0;
+    return lo;
+  };
   if (!accumulate && p->num_empty > 0) {
     const int64_t blocks = (p->num_empty * 64 + 255) / 256;
     hipLaunchKernelGGL(zero_rows_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p->empty_rows,
@@ -498,6 +517,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
         a.rpw = clamp_rpw(t.rpw_long > 0 ? t.rpw_long : 1, S);
         a.xcd_remap = t.xcd_remap ? 1 : 0;
         a.phase = 0;
+        a.abase = table_base(c0, w);
         const int64_t waves = (n + a.rpw - 1) / a.rpw;
         int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bpc);
         blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
@@ -555,6 +575,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
         a.rpw = clamp_rpw(req, S);
         a.xcd_remap = t.xcd_remap ? 1 : 0;
         a.phase = phase;
+        a.abase = table_base(c0, w);
         const int64_t waves = (bin->n + a.rpw - 1) / a.rpw;
         const int bin_bpc = bi == 1 && t.blocks_per_cu_short > 0 ? t.blocks_per_cu_short : bpc;
         int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bin_bpc);
