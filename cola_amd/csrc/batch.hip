@@ -50,6 +50,7 @@ struct BatchSeg {
   int64_t pstride;
   int32_t n_items, dim, rpw, n_tiles, tile_w, g_log2, task_begin, task_end;
   int32_t n_grp;  // row groups of rpw items: tasks = n_grp x n_tiles
+  uint64_t abase; // common base of B and Bp for 32-bit row offsets (RowArgs::abase), 0: none
 };
 
 struct HubSeg {
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256) void batch_rows_kernel(const BatchSeg *__restr
     a.rpw = sg.rpw;
     a.xcd_remap = 0;
     a.phase = 0;
-    a.abase = 0;  // segments keep full 64-bit row addresses
+    a.abase = sg.abase;
     const int it0 = grp * sg.rpw;
     // The lane-group width is wave-uniform (one segment per task): each case
     // is the plan kernel's step with that width compiled in.
@@ -259,6 +260,25 @@ static cola_status append_problem(const cola_batch_problem &pr, const cola_batch
     while (r * 2 <= req && r * 2 <= S) r *= 2;
     return r;
   };
+  // Common base of the problem's two row tables when every addressable row
+  // (local rows below nodesPerPE, packed rows below U_r, all columns) lies
+  // within 2^36 bytes of it on 16-byte boundaries (RowArgs::abase).
+  uint64_t abase = 0;
+  {
+    const uint64_t eb = dtype_bytes(dtype);
+    uint64_t lo = UINT64_MAX, hi = 0;
+    bool ok = true;
+    auto span = [&](const void *base, int64_t rows, int64_t ld) {
+      if (!base || rows <= 0) return;
+      const uint64_t bb = (uint64_t)(uintptr_t)base;
+      ok = ok && bb % 16 == 0 && ((uint64_t)ld * eb) % 16 == 0;
+      lo = std::min(lo, bb);
+      hi = std::max(hi, bb + ((uint64_t)(rows - 1) * (uint64_t)ld + (uint64_t)pr.dim) * eb);
+    };
+    span(pr.B, p->nodesPerPE, pr.ld_B);
+    span(pr.packed, p->U_r, pr.ld_packed);
+    if (ok && lo != UINT64_MAX && hi - lo < (uint64_t(1) << 36)) abase = lo;
+  }
   const PlanBin *bins[3] = {&p->longb, &p->shortb, (flags & COLA_OVERWRITE) ? &p->emptyb : nullptr};
   const int rpws[3] = {clamp(t.rpw_long > 0 ? t.rpw_long : 1), clamp(t.rpw_short > 0 ? t.rpw_short : S), S};
   for (int bi = 0; bi < 3; ++bi) {
@@ -290,6 +310,7 @@ static cola_status append_problem(const cola_batch_problem &pr, const cola_batch
     *task += tasks;
     if (*task > INT32_MAX - (1 << 24)) return fail(COLA_ERR_INVALID, "batch: too many tasks");
     s.task_end = (int32_t)*task;
+    s.abase = abase;
     segs->push_back(s);
     costs->push_back(SegCost{&bin->cost, s.task_begin, s.task_end, s.n_grp, s.rpw, s.tile_w, s.dim, s.n_items});
   }
