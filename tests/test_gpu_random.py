@@ -28,6 +28,20 @@ def rel_fro(C, R):
     return d / r if r > 0 else d
 
 
+def assert_accumulated(got, fill, ref, scale=1, roundings=1):
+    """got = fill + scale * ref, accumulated into an FP32 C prefilled with
+    fill: the product within TOL relative Frobenius, plus half an ulp of
+    |fill + scale * ref| per element for each FP32 add onto C (roundings of
+    them), which no kernel avoids when the product is small next to what C
+    already holds (0.75 + 1e-4 carries 3e-8 of rounding, 3e-4 of the 1e-4)."""
+    got = np.asarray(got, np.float64)
+    want = fill + scale * ref
+    ulp = np.spacing(np.abs(want).astype(np.float32)).astype(np.float64)
+    err = np.linalg.norm(got - want)
+    bound = TOL * scale * np.linalg.norm(ref) + roundings * 0.5 * np.linalg.norm(ulp)
+    assert err <= bound, (err, bound)
+
+
 def dequant(Bq, sc, dtype):
     h = Bq.cpu().numpy()
     if dtype == cola.F32:
@@ -99,7 +113,10 @@ def test_plan_path_random(g, nGPU, data, dim, dtype, opts, tuning, flags):
     rp, ci, va = g.arrays()
     ref = orc.spmm_f64(rp, ci, va, rank * P, rank * P + M, Bd[:n])
     got = C.cpu().numpy()
-    assert rel_fro(got[:M] - (0.0 if overwrite else fill), ref) <= TOL
+    if overwrite:
+        assert rel_fro(got[:M], ref) <= TOL
+    else:  # at most three adds onto C per row: local, remote (split) and cold items
+        assert_accumulated(got[:M], fill, ref, roundings=3)
     assert np.all(got[M:] == fill)  # rows past the partition are never touched
 
 
@@ -172,7 +189,7 @@ def test_reference_api_random(g, nGPU, data, dim, split):
     cola.dist_spmm_ppl_batch(C, B, packed, zl, zr, dim, P, rank)
     torch.cuda.synchronize()
     got = C.cpu().numpy()
-    assert rel_fro((got[:M] - 0.25) / 2, ref) <= TOL
+    assert_accumulated(got[:M], 0.25, ref, scale=2, roundings=2)  # two calls, each one add onto C
     assert np.all(got[M:] == 0.25)
     C.fill_(0.25)
     uniq = torch.from_numpy(np.unique(np.asarray(tr, np.int32))).to(dev)
@@ -180,4 +197,5 @@ def test_reference_api_random(g, nGPU, data, dim, split):
     cola.ppl0_batch_SpMM(C, B[rank * P:(rank + 1) * P], packed, zl, zr, 0, 0, zl.num, zr.num, dim, P, rank)
     torch.cuda.synchronize()
     got = C.cpu().numpy()
-    assert rel_fro(got[:M] - 0.25, ref) <= TOL
+    # one atomic add onto C per (virtual) row segment of each CSR / CSC entry run
+    assert_accumulated(got[:M], 0.25, ref, roundings=max(1, int(np.diff(rp).max(initial=0))))
