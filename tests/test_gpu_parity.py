@@ -430,7 +430,9 @@ def test_quantize_is_bitexact():
     exp_codes = orc.fp8_from_f32((x / exp_sc[:, None]).astype(np.float32))
     assert np.array_equal(f8.cpu().numpy(), exp_codes)
     h, _ = cola.quantize_rows(xt, cola.F16)
-    assert np.array_equal(h.cpu().numpy().view(np.float16), x.astype(np.float16))
+    with np.errstate(over="ignore"):  # 3.0e38 overflows FP16 to inf on both sides
+        want = x.astype(np.float16)
+    assert np.array_equal(h.cpu().numpy().view(np.float16), want)
 
 
 # Error bounds of the reduced-precision storage forms against the FP64 oracle
