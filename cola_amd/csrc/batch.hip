@@ -664,9 +664,17 @@ struct Resident {
 
 struct ReqState {
   int64_t pool_off = -1, pool_bytes = 0;
+  bool live = false;  // holds pool memory
   float *C = nullptr;
   float *partials = nullptr;
 };
+
+// Pool bytes of one request: its output (unless the caller gives one) and
+// its hub partials, each part on a 256-byte boundary, as pool_alloc takes it.
+int64_t round256(int64_t b) { return (b + 255) / 256 * 256; }
+int64_t request_pool_bytes(int64_t out_bytes, int64_t hub_bytes, bool own_output) {
+  return round256((own_output ? round256(out_bytes) : 0) + hub_bytes);
+}
 
 double now_us() {
   return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -686,7 +694,7 @@ struct cola_server_s {
 };
 
 static int64_t pool_alloc(cola_server_s *sv, int64_t bytes) {
-  bytes = (bytes + 255) / 256 * 256;
+  bytes = round256(bytes);
   for (auto it = sv->free_blocks.begin(); it != sv->free_blocks.end(); ++it) {
     if (it->second >= bytes) {
       const int64_t off = it->first, rest = it->second - bytes;
@@ -701,7 +709,7 @@ static int64_t pool_alloc(cola_server_s *sv, int64_t bytes) {
 }
 
 static void pool_free(cola_server_s *sv, int64_t off, int64_t bytes) {
-  bytes = (bytes + 255) / 256 * 256;
+  bytes = round256(bytes);
   sv->used -= bytes;
   auto it = sv->free_blocks.emplace(off, bytes).first;
   auto nx = std::next(it);
@@ -825,10 +833,13 @@ cola_status cola_server_run_trace(cola_server sv, int64_t n, const double *arriv
     if (resident[i] < 0 || resident[i] >= (int)sv->res.size())
       return fail(COLA_ERR_INVALID, "server_run_trace: unknown resident id");
     if (i && arrival_us[i] < arrival_us[i - 1]) return fail(COLA_ERR_INVALID, "server_run_trace: arrivals not sorted");
+    // Checked as admission takes it: a request larger than the whole pool
+    // would wait for memory forever.
     const Resident &r = sv->res[resident[i]];
-    const int64_t need = (outputs && outputs[i] ? 0 : r.out_bytes) + r.hub_bytes;
-    if (need > sv->pool_size) return fail(COLA_ERR_NOMEM, "server_run_trace: a request exceeds the memory budget");
+    if (request_pool_bytes(r.out_bytes, r.hub_bytes, !(outputs && outputs[i])) > sv->pool_size)
+      return fail(COLA_ERR_NOMEM, "server_run_trace: a request exceeds the memory budget");
   }
+  for (int64_t i = 0; i < n; ++i) records[i] = cola_request_record{};
   const int flags = COLA_OVERWRITE | (sv->o.flags & COLA_KAHAN);
   std::vector<ReqState> rs(n);
   std::deque<int64_t> pending;
@@ -841,14 +852,15 @@ cola_status cola_server_run_trace(cola_server sv, int64_t n, const double *arriv
     const Resident &r = sv->res[resident[i]];
     ReqState &q = rs[i];
     const bool own = !(outputs && outputs[i]);
-    q.pool_bytes = (own ? (r.out_bytes + 255) / 256 * 256 : 0) + r.hub_bytes;
+    q.pool_bytes = own || r.hub_bytes > 0 ? request_pool_bytes(r.out_bytes, r.hub_bytes, own) : 0;
     if (q.pool_bytes > 0) {
       q.pool_off = pool_alloc(sv, q.pool_bytes);
       if (q.pool_off < 0) return false;
+      q.live = true;
     }
     q.C = own ? reinterpret_cast<float *>(sv->pool + q.pool_off) : outputs[i];
     q.partials = r.hub_bytes > 0
-                     ? reinterpret_cast<float *>(sv->pool + q.pool_off + (own ? (r.out_bytes + 255) / 256 * 256 : 0))
+                     ? reinterpret_cast<float *>(sv->pool + q.pool_off + (own ? round256(r.out_bytes) : 0))
                      : nullptr;
     return true;
   };
@@ -908,7 +920,8 @@ cola_status cola_server_run_trace(cola_server sv, int64_t n, const double *arriv
       const double tdone = now_us() - t0;
       for (int64_t i : sl.reqs) {
         records[i].done_us = tdone;
-        if (rs[i].pool_bytes > 0) pool_free(sv, rs[i].pool_off, rs[i].pool_bytes);
+        if (rs[i].live) pool_free(sv, rs[i].pool_off, rs[i].pool_bytes);
+        rs[i].live = false;
         ++done;
       }
       sl.busy = false;
@@ -960,8 +973,7 @@ cola_status cola_server_run_trace(cola_server sv, int64_t n, const double *arriv
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
   if (st) {
     for (int64_t i = 0; i < n; ++i)
-      if (rs[i].pool_off >= 0 && records[i].done_us == 0.0 && rs[i].pool_bytes > 0)
-        pool_free(sv, rs[i].pool_off, rs[i].pool_bytes);
+      if (rs[i].live) pool_free(sv, rs[i].pool_off, rs[i].pool_bytes);
     for (auto &sl : sv->slots) {
       sl.busy = false;
       sl.reqs.clear();

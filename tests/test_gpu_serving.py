@@ -102,3 +102,36 @@ def test_request_larger_than_budget_is_rejected(residents):
     sv, ids = make_server(residents, budget=1 << 12)
     with pytest.raises(cola.ColaError):
         sv.run_trace([0.0], [ids[0]])
+
+
+def test_budget_check_counts_the_rounding_admission_uses():
+    """A request is sized as admission takes it (output and hub partials each
+    on a 256-byte boundary): a budget that holds only its unrounded bytes is
+    rejected up front instead of leaving the request waiting for memory, and
+    a budget of the rounded size serves it."""
+    r256 = lambda b: (b + 255) // 256 * 256  # noqa: E731
+    n, dim, max_dim = 9, 8, 16
+    rp = np.array([0, 8] + list(range(9, 17)), np.int64)  # row 0 -> rows 1..8, each of them -> row 0
+    ci = np.array(list(range(1, 9)) + [0] * 8, np.int32)
+    g = cola.Graph.from_csr(rp, ci, np.ones(16, np.float32))
+    plan = cola.Plan(g.partition(1, 0), hub_threshold=4, chunk_nnz=4, max_dim=max_dim, short_degree=2)
+    chunks = plan.info["num_hub_chunks"]
+    assert chunks == 2
+    out_bytes = n * dim * 4
+    hub_bytes = chunks * ((max_dim + 15) // 16 * 16) * 4  # partial rows of max_dim rounded to 16 floats
+    rounded = r256(r256(out_bytes) + hub_bytes)
+    assert r256(out_bytes + hub_bytes) < rounded  # the case that tells the two sizings apart
+    B = torch.rand((n, dim), device="cuda").to(torch.bfloat16)
+    small = cola.Server(out_bytes + hub_bytes, dtype=cola.BF16)
+    rid = small.add_resident(plan, B)
+    with pytest.raises(cola.ColaError):
+        small.run_trace([0.0], [rid])
+    fits = cola.Server(rounded, dtype=cola.BF16)
+    rid = fits.add_resident(plan, B)
+    out = torch.full((n, dim), -1.0, device="cuda")
+    recs, _, peak = fits.run_trace([0.0, 10.0], [rid, rid], outputs=[out, None])
+    assert peak <= rounded and np.all(recs["done_us"] > 0)
+    ref = torch.zeros_like(out)
+    plan.spmm(ref, B, None, dim=dim, dtype=cola.BF16, flags=cola.OVERWRITE)
+    torch.cuda.synchronize()
+    assert torch.allclose(out, ref, rtol=0, atol=1e-5)
