@@ -356,7 +356,14 @@ static cola_status launch_tables(const BatchSeg *d_segs, int n_segs, int total_t
 }
 
 // Rebuilds the segment / hub tables from the problems and their tunings.
+// Tables already built may be in use by a launch still running on any
+// stream (non-blocking streams do not order against hipMemcpy), so the device
+// is drained before they are rewritten or the captured graph is dropped.
 static cola_status build_tables(cola_batch_s *b) {
+  if (b->n_segs > 0 || b->n_hubs > 0 || b->exec) {
+    cola_status st = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize before a table rebuild");
+    if (st) return st;
+  }
   drop_graph(b);
   std::vector<BatchSeg> segs;
   std::vector<SegCost> costs;

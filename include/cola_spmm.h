@@ -431,7 +431,9 @@ typedef struct {
 cola_status cola_batch_create(const cola_batch_problem *probs, int n,
                               cola_dtype dtype, int flags, cola_batch *out);
 /* Sets one problem's schedule (problem = -1: all) and/or the grid size
- * (blocks_per_cu > 0); rebuilds the work list and drops a captured graph. */
+ * (blocks_per_cu > 0); waits for the device, then rebuilds the work list and
+ * drops a captured graph.  The batch's hub workspace is shared by its runs:
+ * runs of one batch on different streams at the same time race. */
 cola_status cola_batch_set_tuning(cola_batch b, int problem,
                                   const cola_batch_tuning *t, int blocks_per_cu);
 cola_status cola_batch_get_tuning(cola_batch b, int problem,
