@@ -487,26 +487,54 @@ def test_error_norms_on_device(c1):
     assert np.isclose(out[2], ((d * d).sum(1)[nz] / rn[nz]).max(), rtol=1e-10)
 
 
-def test_full_size_partition_checksum():
-    """The 2^22-node partition (R-MAT, degree 32, Cuthill-McKee order,
-    partition 0 of 8, dim 128): row sums of C equal A_p·(B·1) computed by the
-    FP64 oracle (a size-independent checksum), and a sample of rows matches
-    the FP64 product."""
-    g = cola.Graph.rmat(22, 32, seed=2, reorder=True)
-    rp, ci, va = g.arrays()
+@pytest.fixture(scope="module", params=["cm", "community"])
+def full_size(request):
+    """The 2^22-node R-MAT graph (degree 32, seed 2) in Cuthill-McKee or
+    community order, partition 0 of 8, B of 2^22 x 128 FP32 uniform[-1,1]."""
+    order = True if request.param == "cm" else "community"
+    g = cola.Graph.rmat(22, 32, seed=2, reorder=order)
     part = g.partition(8, 0)
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    P = part.info()["nodesPerPE"]
+    B = torch.rand((P * 8, 128), generator=gen, device="cuda", dtype=torch.float32) * 2 - 1
+    return dict(order=request.param, g=g, part=part, B=B)
+
+
+# The large-graph default schedule and the benchmarked one (256-byte tiles,
+# 16 blocks per CU on the short-row bin).
+FULL_SIZE_TUNINGS = [None, dict(col_tile=64, rpw_short=0, rpw_long=4, blocks_per_cu=2, xcd_remap=1,
+                                split_remote=0, unroll=8, panels=0, blocks_per_cu_short=16)]
+
+
+@pytest.mark.parametrize("tuning", FULL_SIZE_TUNINGS, ids=["default", "bench"])
+def test_full_size_partition_checksum(full_size, tuning):
+    """Partition 0 of 8 of the 2^22-node graph at dim 128: row sums of C equal
+    A_p·(B·1) from the FP64 oracle (a size-independent checksum), a sample of
+    rows matches the FP64 product, and doubling B doubles C exactly (every
+    product and sum scales by a power of two)."""
+    g, part, B = full_size["g"], full_size["part"], full_size["B"]
+    rp, ci, va = g.arrays()
     info = part.info()
     P, M, dim = info["nodesPerPE"], info["num_rows"], 128
-    gen = torch.Generator(device="cuda").manual_seed(5)
-    B = torch.rand((P * 8, dim), generator=gen, device="cuda", dtype=torch.float32) * 2 - 1
-    plan = cola.Plan(part)
+    print(f"case full size order={full_size['order']} nnz={info['nnz_l'] + info['nnz_r']} tuning={tuning}",
+          flush=True)
+    plan = cola.Plan(part, row_order=1)
+    if tuning is not None:
+        plan.tuning = tuning
     packed = torch.empty((plan.U_r, dim), dtype=torch.float32, device="cuda")
     C = torch.zeros((P, dim), dtype=torch.float32, device="cuda")
     plan.pack(B, packed)
     plan.spmm(C, B[:P], packed, flags=cola.OVERWRITE)
+    B2 = B * 2
+    packed2 = torch.empty_like(packed)
+    C2 = torch.zeros_like(C)
+    plan.pack(B2, packed2)
+    plan.spmm(C2, B2[:P], packed2, flags=cola.OVERWRITE)
+    torch.cuda.synchronize()
+    assert torch.equal(C2, C * 2)
+    del B2, packed2, C2
     rowsum_B = B.double().sum(1).cpu().numpy()
     rowsum_C = C.double().sum(1).cpu().numpy()[:M]
-    torch.cuda.synchronize()
     ref = orc.spmm_f64(rp, ci, va, 0, M, rowsum_B.astype(np.float32)[:, None])[:, 0]
     assert np.linalg.norm(rowsum_C - ref) / np.linalg.norm(ref) <= 1e-5
     sample = np.unique(np.concatenate([np.arange(0, M, 997), [0, 1, M - 1]]))
