@@ -544,6 +544,44 @@ def test_full_size_partition_checksum(full_size, tuning):
     assert rel_fro(Ch[sample], refs) <= FP32_TOL
 
 
+def test_full_size_reference_entry_points(full_size):
+    """The reference's entry points at full size (zips through nnz_split, as
+    the shipped driver builds them): the single-stage wrapper rebuilds the
+    default decreasing-degree plan from the zips, so it equals the plan path
+    bit for bit; the literal virtual-row kernel with FP32 atomics matches the
+    FP64 oracle on sampled rows."""
+    g, part, B = full_size["g"], full_size["part"], full_size["B"]
+    rp, ci, va = g.arrays()
+    info = part.info()
+    P, M, dim = info["nodesPerPE"], info["num_rows"], 128
+    print(f"case full size reference entry points order={full_size['order']}", flush=True)
+    a = part.arrays()
+    tl, pl, _ = cola.nnz_split(a["csr_rowIdx_l"], a["csr_rowPtr_l"], len(a["csr_colIdx_l"]))
+    tr, pr, _ = cola.nnz_split(a["csc_colIdx_r"], a["csc_colPtr_r"], len(a["csc_rowIdx_r"]))
+    zl = cola.d_csr_zip(pl, a["csr_colIdx_l"], tl, a["csr_val_l"])
+    zr = cola.d_csc_zip(pr, a["csc_rowIdx_r"], tr, a["csc_val_r"])
+    packed = torch.empty((max(zr.n_unique, 1), dim), dtype=torch.float32, device="cuda")
+    C_ref_api = torch.zeros((P, dim), dtype=torch.float32, device="cuda")
+    cola.dist_spmm_ppl_batch(C_ref_api, B, packed, zl, zr, dim, P, 0)
+    plan = cola.Plan(part, row_order=1)
+    packed_p = torch.empty((plan.U_r, dim), dtype=torch.float32, device="cuda")
+    C_plan = torch.zeros((P, dim), dtype=torch.float32, device="cuda")
+    plan.pack(B, packed_p)
+    plan.spmm(C_plan, B[:P], packed_p, flags=cola.OVERWRITE)
+    torch.cuda.synchronize()
+    assert torch.equal(C_ref_api, C_plan)
+    del C_plan, packed_p
+    C_lit = torch.zeros((P, dim), dtype=torch.float32, device="cuda")
+    uniq = torch.from_numpy(np.unique(np.asarray(tr, np.int32))).cuda()
+    cola.pack_remote(B, uniq, packed, num_cols=zr.n_unique)
+    cola.ppl0_batch_SpMM(C_lit, B[:P], packed, zl, zr, 0, 0, zl.num, zr.num, dim, P, 0)
+    torch.cuda.synchronize()
+    sample = np.unique(np.concatenate([np.arange(0, M, 997), [0, 1, M - 1]]))
+    Bh = B.cpu().numpy()
+    refs = np.stack([orc.spmm_f64(rp, ci, va, int(i), int(i) + 1, Bh[:g.n])[0] for i in sample])
+    assert rel_fro(C_lit.cpu().numpy()[sample], refs) <= FP32_TOL
+
+
 TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=8, panels=0),
            dict(col_tile=32, rpw_short=1, rpw_long=2, blocks_per_cu=2, xcd_remap=0, split_remote=1),
            dict(col_tile=16, rpw_short=2, rpw_long=4, blocks_per_cu=16, xcd_remap=1, split_remote=0),
