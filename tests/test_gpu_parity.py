@@ -582,6 +582,63 @@ def test_full_size_reference_entry_points(full_size):
     assert rel_fro(C_lit.cpu().numpy()[sample], refs) <= FP32_TOL
 
 
+def sampled_rows_csr(rp, ci, va, rows):
+    """CSR of the given rows alone, columns renumbered over the B rows they
+    reference (returned as `cols`), so an FP64 product over them needs only
+    those B rows."""
+    rp = np.asarray(rp, np.int64)
+    idx = np.concatenate([np.arange(rp[i], rp[i + 1]) for i in rows])
+    cols, inv = np.unique(np.asarray(ci)[idx], return_inverse=True)
+    lens = np.array([rp[i + 1] - rp[i] for i in rows], np.int64)
+    rp2 = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    return rp2, inv.astype(np.int32), np.asarray(va, np.float32)[idx], cols
+
+
+@pytest.mark.parametrize("dtype", [cola.BF16, cola.F16, cola.FP8_E4M3], ids=["bf16", "f16", "fp8"])
+def test_full_size_precision_forms(full_size, dtype):
+    """B stored in BF16 / FP16 / FP8-e4m3 (+ row scales) on the 2^22-node
+    partition, FP32 accumulation with Kahan on rows above 1024 entries: on
+    527 sampled rows the result equals the FP64 product of the dequantised B
+    to FP32 accuracy, and is within the storage format's error of the FP64
+    product of the FP32 B."""
+    g, part, B = full_size["g"], full_size["part"], full_size["B"]
+    rp, ci, va = g.arrays()
+    info = part.info()
+    P, M, dim = info["nodesPerPE"], info["num_rows"], 128
+    print(f"case full size precision order={full_size['order']} dtype={cola.DTYPE_NAMES[dtype]}", flush=True)
+    Bq, sc = cola.quantize_rows(B, dtype)
+    plan = cola.Plan(part, row_order=1)
+    tdt = {1: torch.uint8, 2: torch.uint16}[cola.DTYPE_BYTES[dtype]]
+    packed = torch.empty((plan.U_r, dim), dtype=tdt, device="cuda")
+    psc = torch.empty(plan.U_r, dtype=torch.float32, device="cuda") if dtype == cola.FP8_E4M3 else None
+    plan.pack(Bq, packed, dim=dim, dtype=dtype, scales_global=sc, packed_scales=psc)
+    C = torch.zeros((P, dim), dtype=torch.float32, device="cuda")
+    plan.spmm(C, Bq[:P], packed, dim=dim, dtype=dtype, flags=cola.OVERWRITE | cola.KAHAN,
+              scales_local=None if sc is None else sc[:P], packed_scales=psc)
+    torch.cuda.synchronize()
+    sample = np.unique(np.concatenate([np.arange(0, M, 997), [0, 1, M - 1]]))
+    rp2, ci2, va2, cols = sampled_rows_csr(rp, ci, va, sample)
+    cols_t = torch.from_numpy(cols.astype(np.int64)).cuda()
+    Bf = B[cols_t].cpu().numpy()
+    if Bq.dtype == torch.uint16:  # torch gathers no uint16 on the GPU: go through int16
+        q = Bq.view(torch.int16)[cols_t].cpu().numpy().view(np.uint16)
+    else:
+        q = Bq[cols_t].cpu().numpy()
+    if dtype == cola.FP8_E4M3:
+        deq = orc.fp8_to_f32(q) * sc[cols_t].cpu().numpy()[:, None]
+    elif dtype == cola.BF16:
+        deq = (q.astype(np.uint32) << 16).view(np.float32)
+    else:
+        deq = q.view(np.float16).astype(np.float32)
+    got = C.cpu().numpy()[sample]
+    ref = orc.spmm_f64(rp2, ci2, va2, 0, len(sample), Bf)
+    ref_q = orc.spmm_f64(rp2, ci2, va2, 0, len(sample), deq.astype(np.float32))
+    err, err_q = rel_fro(got, ref), rel_fro(got, ref_q)
+    print(f"rel_fro_vs_fp64={err:.3e} vs_dequantised={err_q:.3e}", flush=True)
+    assert err_q <= FP32_TOL
+    assert err <= PREC_TOL[dtype]
+
+
 TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=8, panels=0),
            dict(col_tile=32, rpw_short=1, rpw_long=2, blocks_per_cu=2, xcd_remap=0, split_remote=1),
            dict(col_tile=16, rpw_short=2, rpw_long=4, blocks_per_cu=16, xcd_remap=1, split_remote=0),
