@@ -164,3 +164,35 @@ def test_batch_rejects_bad_dims():
     probs[0]["dim"] = 36  # not a multiple of 8 bf16 elements
     with pytest.raises(cola.ColaError):
         cola.Batch(probs, dtype=cola.BF16)
+
+
+def test_batched_configuration_64_problems():
+    """The benchmarked batch itself (64 SpMMs, node count log-uniform in
+    [2^12, 2^18], degree uniform in [4, 64], dim from {32, 64, 128, 256}, B in
+    BF16, items by decreasing degree), autotuned and replayed from a hipGraph
+    twice: every problem matches its own launch, and every eighth problem
+    matches the FP64 product of its dequantised B on sampled rows."""
+    from cola_amd import workloads
+    specs = workloads.batched_specs(64, seed=7)
+    probs = workloads.build_problems(specs, dtype=cola.BF16, plan_opts=dict(row_order=1))
+    print(f"case batched configuration: {sum(p['nnz'] for p in probs)} entries", flush=True)
+    refs = [individual(p, cola.BF16) for p in probs]
+    batch = cola.Batch(probs, dtype=cola.BF16, flags=cola.OVERWRITE)
+    batch.autotune(iters=2)
+    s = torch.cuda.Stream()
+    batch.capture(s)
+    for rep in range(2):
+        for p in probs:
+            p["C"].fill_(float(rep) - 3.0)
+        torch.cuda.synchronize()
+        batch.replay(s)
+        s.synchronize()
+        for k, (p, r) in enumerate(zip(probs, refs)):
+            assert rel_fro(p["C"].cpu().numpy(), r.cpu().numpy().astype(np.float64)) <= 1e-6, (rep, k)
+    for p in probs[::8]:
+        n = p["n"]
+        rows = np.unique(np.concatenate([np.arange(0, n, 211), [n - 1]]))
+        rp, ci, va = p["graph"].arrays()
+        deq = dequant(p["B"], None, cola.BF16)
+        ref = np.stack([orc.spmm_f64(rp, ci, va, int(i), int(i) + 1, deq)[0] for i in rows])
+        assert rel_fro(p["C"].cpu().numpy()[rows], ref) <= 1e-5
