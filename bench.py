@@ -531,6 +531,10 @@ def main():
                      "kernel_ms": round(spmm_ms, 4), "kernel_bytes_alg": kbytes},
         "cpu_baseline": cpu,
         "kernels_ms": {"pack": round(pack_ms, 4), "spmm": round(spmm_ms, 4)},
+        # Context, not roofline: FLOP rate of the step, and the rate the
+        # row kernel would need if every entry's B row came from HBM.
+        "context": {"gflops_step": round(2 * nnz_p * N / (ms * 1e-3) / 1e9, 1),
+                    "gather_no_reuse_GBps": round(nnz_p * N * s_B / (spmm_ms * 1e-3) / 1e9, 1)},
         "extras": extras,
     }
     print(json.dumps(out), flush=True)

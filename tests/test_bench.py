@@ -37,6 +37,7 @@ def test_bench_json_line_on_a_small_graph():
     r = d["roofline"]
     assert r["bound"] == "hbm" and r["peak"] == 8000.0 and r["unit"] == "GB/s"
     assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-3
+    assert d["context"]["gflops_step"] > 0 and d["context"]["gather_no_reuse_GBps"] > 0
     c = d["cpu_baseline"]
     assert c["kind"] == "port" and c["value"] > 0 and c["cores"] >= 1
     assert d["config"]["workload"].startswith("rmat14_deg8_community_part0of8")
