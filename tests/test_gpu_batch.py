@@ -166,6 +166,41 @@ def test_batch_rejects_bad_dims():
         cola.Batch(probs, dtype=cola.BF16)
 
 
+
+def test_capture_errors_leave_direct_launches_working():
+    """A capture that cannot happen (on the default stream) and a replay
+    with nothing captured return errors, and direct launches keep working; a
+    tuning change drops a captured graph, so replay errors until the batch is
+    captured again."""
+    probs = make_problems(4, cola.BF16, seed=9)
+    batch = cola.Batch(probs, dtype=cola.BF16)
+    with pytest.raises(cola.ColaError):
+        batch.capture(0)
+    with pytest.raises(cola.ColaError):
+        batch.replay()
+    batch.run()
+    torch.cuda.synchronize()
+    ref = [p["C"].clone() for p in probs]
+    s = torch.cuda.Stream()
+    batch.capture(s)
+    batch.set_tuning(rpw_short=2)
+    with pytest.raises(cola.ColaError):
+        batch.replay(s)
+    for p in probs:
+        p["C"].fill_(1.0)
+    batch.run()
+    torch.cuda.synchronize()
+    for p, r in zip(probs, ref):
+        assert rel_fro(p["C"].cpu().numpy(), r.cpu().numpy().astype(np.float64)) <= 1e-6
+    batch.capture(s)
+    for p in probs:
+        p["C"].fill_(2.0)
+    torch.cuda.synchronize()
+    batch.replay(s)
+    s.synchronize()
+    for p, r in zip(probs, ref):
+        assert rel_fro(p["C"].cpu().numpy(), r.cpu().numpy().astype(np.float64)) <= 1e-6
+
 def test_batched_configuration_64_problems():
     """The benchmarked batch itself (64 SpMMs, node count log-uniform in
     [2^12, 2^18], degree uniform in [4, 64], dim from {32, 64, 128, 256}, B in

@@ -166,6 +166,15 @@ def test_load_error_paths(tmp_path):
         cola.Partition.load(str(tmp_path), "g", 2, 0)
     os.remove(d / "csr_val_l_2_0.bin")
     cola.Partition.load(str(tmp_path), "g", 2, 0)
+    # any index file written as int64 (numpy's default integer) instead of int32
+    for stem in ("csr_rowPtr_l", "csr_colIdx_l", "csr_rowIdx_l", "csc_colPtr_r", "csc_rowIdx_r", "csc_colIdx_r"):
+        f = d / f"{stem}_2_0.bin"
+        good = np.fromfile(f, np.int32)
+        good.astype(np.int64).tofile(f)
+        with pytest.raises(ColaError):
+            cola.Partition.load(str(tmp_path), "g", 2, 0)
+        good.tofile(f)
+    cola.Partition.load(str(tmp_path), "g", 2, 0)
     # rank out of range
     with pytest.raises(ColaError):
         cola.Partition.load(str(tmp_path), "g", 2, 2)
