@@ -45,6 +45,30 @@ def _stream(stream):
     return ctypes.c_void_p(stream.cuda_stream)
 
 
+def _check_matrix(t, name, rows, dim, elem_bytes):
+    """A row-major GPU matrix the kernels will index: at least `rows` rows
+    of `dim` used columns, unit column stride, elements of `elem_bytes`
+    bytes.  The C ABI takes raw pointers and cannot check this itself."""
+    if t is None:
+        if rows > 0:
+            raise ValueError(f"{name} is required ({rows} rows)")
+        return
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a GPU tensor")
+    if t.element_size() != elem_bytes:
+        raise ValueError(f"{name} has {t.element_size()}-byte elements; the storage type needs {elem_bytes}")
+    if rows > 0:
+        if t.dim() != 2 or t.stride(1) != 1:
+            raise ValueError(f"{name} must be a 2-D tensor with contiguous rows")
+        if t.shape[0] < rows or t.shape[1] < dim:
+            raise ValueError(f"{name} is {tuple(t.shape)}; needs at least ({rows}, {dim})")
+
+
+def _check_vector(t, name, n):
+    if n > 0 and (t is None or not t.is_cuda or t.element_size() != 4 or t.numel() < n):
+        raise ValueError(f"{name} must be a GPU float32 vector of at least {n} entries")
+
+
 def _host(a, dtype):
     a = np.ascontiguousarray(np.asarray(a), dtype=dtype)
     return a
@@ -179,13 +203,32 @@ class Plan:
             _lib._lib.cola_plan_destroy(self._h)
             self._h = None
 
+    def _check_call(self, C, B_local, packed, dim, dtype, scales_local, packed_scales):
+        if dtype not in DTYPE_BYTES:
+            raise ValueError(f"unknown storage dtype {dtype}")
+        eb = DTYPE_BYTES[dtype]
+        _check_matrix(C, "C", self.num_rows, dim, 4)
+        _check_matrix(B_local, "B_local", self.num_rows if self.U_l > 0 else 0, dim, eb)
+        _check_matrix(packed, "packed", self.U_r, dim, eb)
+        if dtype == FP8_E4M3:
+            _check_vector(scales_local, "scales_local", self.num_rows if self.U_l > 0 else 0)
+            _check_vector(packed_scales, "packed_scales", self.U_r)
+
     def pack(self, B_global, packed, dim=None, dtype=F32, scales_global=None, packed_scales=None, stream=None):
+        dim = dim if dim is not None else B_global.shape[1]
+        if dtype not in DTYPE_BYTES:
+            raise ValueError(f"unknown storage dtype {dtype}")
+        _check_matrix(packed, "packed", self.U_r, dim, DTYPE_BYTES[dtype])
+        _check_matrix(B_global, "B_global", 1 if self.U_r > 0 else 0, dim, DTYPE_BYTES[dtype])
+        if dtype == FP8_E4M3:
+            _check_vector(packed_scales, "packed_scales", self.U_r)
         pack_remote(B_global, self.remote_cols_ptr, packed, dim=dim, dtype=dtype, scales_global=scales_global,
                     packed_scales=packed_scales, num_cols=self.U_r, stream=stream)
 
     def spmm(self, C, B_local, packed, dim=None, dtype=F32, flags=ACCUMULATE, scales_local=None,
              packed_scales=None, stream=None):
         dim = dim if dim is not None else C.shape[1]
+        self._check_call(C, B_local, packed, dim, dtype, scales_local, packed_scales)
         ldp = packed.stride(0) if packed is not None and packed.dim() == 2 else dim
         check(lib().cola_spmm_partition(self._h, _dptr(C), C.stride(0), _dptr(B_local), B_local.stride(0),
                                         _dptr(scales_local), _dptr(packed), ldp, _dptr(packed_scales),
@@ -211,6 +254,7 @@ class Plan:
         """Times candidate launch parameters on these buffers and keeps the
         fastest; C is overwritten with A_p·B.  Returns (tuning dict, ms)."""
         dim = dim if dim is not None else C.shape[1]
+        self._check_call(C, B_local, packed, dim, dtype, scales_local, packed_scales)
         ldp = packed.stride(0) if packed is not None and packed.dim() == 2 else dim
         best = _lib.PlanTuning()
         ms = ctypes.c_float()
@@ -233,6 +277,8 @@ class Batch:
         for k, pr in enumerate(problems):
             B, C = pr["B"], pr["C"]
             packed = pr.get("packed")
+            pr["plan"]._check_call(C, B, packed, int(pr.get("dim", C.shape[1])), dtype, pr.get("scales"),
+                                   pr.get("packed_scales"))
             arr[k].plan = pr["plan"].handle.value
             arr[k].dim = int(pr.get("dim", C.shape[1]))
             arr[k].B = B.data_ptr() if B is not None else None

@@ -474,6 +474,37 @@ def test_precision_forms(c1, dtype, dim):
     assert err_q <= FP32_TOL
 
 
+def test_python_wrapper_rejects_buffers_the_kernels_would_overrun(c1):
+    """The C ABI takes raw pointers; the Python mirror checks every buffer it
+    hands over (rows, used columns, element size) and raises before any
+    launch."""
+    plan = cola.Plan(c1.part)
+    dim, dev = c1.dim, c1.dev
+    packed = torch.empty((plan.U_r, dim), dtype=torch.float32, device=dev)
+    C = torch.zeros((c1.P, dim), dtype=torch.float32, device=dev)
+    Bl = c1.B_local()
+    bad_calls = [
+        dict(C=C[: c1.M - 1]),                                              # too few output rows
+        dict(C=torch.zeros((c1.P, dim), dtype=torch.float16, device=dev)),  # C must be FP32
+        dict(B_local=Bl[: c1.M - 1]),                                       # too few B rows
+        dict(packed=packed[: plan.U_r - 1]),                                # too few packed rows
+        dict(packed=packed[:, : dim - 1], dim=dim),                         # too few used columns
+        dict(dtype=cola.BF16),                                              # FP32 tensors, BF16 storage
+        dict(B_local=Bl.t()),                                               # column-major
+    ]
+    for bad in bad_calls:
+        args = dict(C=C, B_local=Bl, packed=packed, dim=dim, dtype=cola.F32)
+        args.update(bad)
+        with pytest.raises(ValueError):
+            plan.spmm(args["C"], args["B_local"], args["packed"], dim=args["dim"], dtype=args["dtype"])
+    with pytest.raises(ValueError):
+        plan.pack(c1.B, packed[: plan.U_r - 1])
+    plan.pack(c1.B, packed)
+    plan.spmm(C, Bl, packed, flags=cola.OVERWRITE)
+    torch.cuda.synchronize()
+    assert rel_fro(C.cpu().numpy()[: c1.M], c1.ref) <= FP32_TOL
+
+
 def test_error_norms_on_device(c1):
     C, _ = c1.run_plan()
     Ct = torch.from_numpy(C[:c1.M]).cuda()
