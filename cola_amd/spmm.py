@@ -88,6 +88,9 @@ class _Zip:
         vp = lambda a: a.ctypes.data_as(ctypes.c_void_p) if a is not None and a.size else None  # noqa: E731
         check(lib().cola_zip_create(vp(ptr), len(ptr), vp(idx), len(idx), vp(zid), vp(va), ctypes.byref(h)))
         self._h = h
+        # Largest ids, for the Python-side buffer checks of the entry points.
+        self.max_id = int(zid.max()) if zid.size else -1
+        self.max_idx = int(idx.max()) if idx.size else -1
         self._read()
 
     def _read(self):
@@ -148,6 +151,12 @@ def pack_remote(B_global, remote_cols, packed, dim=None, dtype=F32, scales_globa
     """packed[u, :dim] = B_global[remote_cols[u], :dim] (row strides from the tensors)."""
     dim = dim if dim is not None else B_global.shape[1]
     n = num_cols if num_cols is not None else (remote_cols.numel() if hasattr(remote_cols, "numel") else 0)
+    if dtype not in DTYPE_BYTES:
+        raise ValueError(f"unknown storage dtype {dtype}")
+    _check_matrix(packed, "packed", n, dim, DTYPE_BYTES[dtype])
+    _check_matrix(B_global, "B_global", 1 if n > 0 else 0, dim, DTYPE_BYTES[dtype])
+    if hasattr(remote_cols, "numel") and remote_cols.numel() < n:
+        raise ValueError(f"remote_cols holds {remote_cols.numel()} ids; {n} requested")
     cols_ptr = remote_cols if isinstance(remote_cols, int) else _dptr(remote_cols)
     check(lib().cola_pack_remote(_dptr(B_global), B_global.stride(0), _dptr(scales_global), cols_ptr, int(n),
                                  int(dim), int(dtype), _dptr(packed), packed.stride(0), _dptr(packed_scales),
@@ -157,6 +166,11 @@ def pack_remote(B_global, remote_cols, packed, dim=None, dtype=F32, scales_globa
 def ppl0_batch_SpMM(output, input, packed, csr_zip_l, csc_zip_r, local_row_lb, remote_col_lb,
                     local_row_num, remote_col_num, dim, nodesPerPE, peid, stream=None):
     """Reference-semantics partition SpMM (FP32 atomics into output)."""
+    if local_row_num + remote_col_num > 0:
+        out_rows = max(csr_zip_l.max_id, csc_zip_r.max_idx) + 1
+        _check_matrix(output, "output", out_rows, dim, 4)
+        _check_matrix(input, "input", csr_zip_l.max_idx + 1 if local_row_num > 0 else 0, dim, 4)
+        _check_matrix(packed, "packed", csc_zip_r.n_unique if remote_col_num > 0 else 0, dim, 4)
     check(lib().cola_ppl0_batch_spmm(
         _dptr(output), _dptr(input), _dptr(packed),
         csr_zip_l._ptr, csr_zip_l._idx, csr_zip_l._zip, csr_zip_l._val,
@@ -168,6 +182,10 @@ def ppl0_batch_SpMM(output, input, packed, csr_zip_l, csc_zip_r, local_row_lb, r
 def dist_spmm_ppl_batch(output, B_global, packed, csr_zip_l, csc_zip_r, dim, nodesPerPE, peid, stream=None):
     """Single-stage distSpMM_ppl_batch_wrapper: pack the unique remote rows, then
     accumulate every virtual local row and remote column into output."""
+    _check_matrix(output, "output", max(csr_zip_l.max_id, csc_zip_r.max_idx) + 1, dim, 4)
+    b_rows = max(peid * nodesPerPE + csr_zip_l.max_idx + 1 if csr_zip_l.nnz else 0, csc_zip_r.max_id + 1)
+    _check_matrix(B_global, "B_global", b_rows, dim, 4)
+    _check_matrix(packed, "packed", csc_zip_r.n_unique, dim, 4)
     check(lib().cola_dist_spmm_ppl_batch(_dptr(output), _dptr(B_global), _dptr(packed), csr_zip_l.handle,
                                          csc_zip_r.handle, int(dim), int(nodesPerPE), int(peid),
                                          _stream(stream)))
@@ -351,6 +369,9 @@ class Server:
     def add_resident(self, plan, B, dim=None, scales=None):
         rid = ctypes.c_int()
         dim = dim if dim is not None else B.shape[1]
+        _check_matrix(B, "B", plan.num_rows if plan.U_l > 0 else 0, dim, DTYPE_BYTES[self.dtype])
+        if self.dtype == FP8_E4M3:
+            _check_vector(scales, "scales", plan.num_rows if plan.U_l > 0 else 0)
         check(lib().cola_server_add_resident(self._h, plan.handle, _dptr(B), B.stride(0), _dptr(scales), int(dim),
                                              ctypes.byref(rid)))
         self._keep.append((plan, B, scales))
