@@ -433,7 +433,6 @@ cola_status cola_batch_create(const cola_batch_problem *probs, int n, cola_dtype
   if (!dtype_bytes(dtype)) return fail(COLA_ERR_UNSUPPORTED, "batch_create: unknown dtype");
   if (flags & ~(COLA_OVERWRITE | COLA_KAHAN)) return fail(COLA_ERR_INVALID, "batch_create: unknown flag");
   const int ev = vec_elems(dtype);
-  const size_t eb = dtype_bytes(dtype);
   for (int k = 0; k < n; ++k) {
     const cola_batch_problem &pr = probs[k];
     if (!pr.plan) return fail(COLA_ERR_INVALID, "batch_create: problem without a plan");
@@ -451,7 +450,6 @@ cola_status cola_batch_create(const cola_batch_problem *probs, int n, cola_dtype
     if (p->num_chunks > 0 && pr.dim > p->max_dim) return fail(COLA_ERR_INVALID, "batch_create: dim exceeds max_dim");
     if (p->cshortb.n + p->clongb.n > 0)
       return fail(COLA_ERR_UNSUPPORTED, "batch_create: a hot/cold plan needs two ordered passes");
-    (void)eb;
   }
   auto *b = new (std::nothrow) cola_batch_s;
   if (!b) return fail(COLA_ERR_NOMEM, "batch_create: out of host memory");

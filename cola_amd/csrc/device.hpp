@@ -68,5 +68,5 @@ struct cola_plan_s {
   int32_t *panel_xcd = nullptr, *panel_hub_ptr = nullptr;
   std::vector<int64_t> panel_begin;
   int64_t panel_chunks = 0;
-  cola_plan_tuning tune{0, 0, 1, 8, 1, 0, 8, 0};  // set by size at creation
+  cola_plan_tuning tune{0, 0, 1, 8, 1, 0, 8, 0, 0};  // set by size at creation
 };

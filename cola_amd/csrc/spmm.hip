@@ -112,7 +112,6 @@ __device__ __forceinline__ uint32_t f32_to_fp8e4m3_rne(float f) {
 }
 
 
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // ---------------------------------------------------------------------------
 // Gather-pack: packed[u] = B[cols[u]].  A group of lanes copies one row with
