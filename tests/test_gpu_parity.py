@@ -505,6 +505,48 @@ def test_python_wrapper_rejects_buffers_the_kernels_would_overrun(c1):
     assert rel_fro(C.cpu().numpy()[: c1.M], c1.ref) <= FP32_TOL
 
 
+def test_two_host_threads_with_their_own_plans_and_streams(c1):
+    """Handles are re-entrant per handle: two host threads, each with its own
+    plan, buffers and stream, call pack + SpMM concurrently (ctypes drops the
+    GIL around each call); both keep matching the FP64 oracle, and a bad call
+    on one thread leaves its own error text without disturbing the other."""
+    import threading
+    errors = []
+
+    def worker(k):
+        try:
+            torch.cuda.set_device(0)
+            s = torch.cuda.Stream()
+            plan = cola.Plan(c1.part, row_order=k)
+            packed = torch.empty((plan.U_r, c1.dim), dtype=torch.float32, device=c1.dev)
+            C = torch.empty((c1.P, c1.dim), dtype=torch.float32, device=c1.dev)
+            with torch.cuda.stream(s):
+                for it in range(20):
+                    plan.pack(c1.B, packed, stream=s)
+                    plan.spmm(C, c1.B_local(), packed, flags=cola.OVERWRITE, stream=s)
+                    if k == 1 and it == 10:
+                        try:
+                            plan.spmm(C, c1.B_local(), packed, dim=c1.dim, dtype=cola.F32, flags=64, stream=s)
+                        except cola.ColaError as e:
+                            assert "unknown flag" in str(e)
+                        else:
+                            raise AssertionError("a bad flag was accepted")
+                s.synchronize()
+            err = rel_fro(C.cpu().numpy()[: c1.M], c1.ref)
+            if err > FP32_TOL:
+                errors.append((k, err))
+        except Exception as e:  # reported to the main thread
+            errors.append((k, repr(e)))
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in (0, 1)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not any(t.is_alive() for t in threads)
+    assert not errors, errors
+
+
 def test_error_norms_on_device(c1):
     C, _ = c1.run_plan()
     Ct = torch.from_numpy(C[:c1.M]).cuda()
