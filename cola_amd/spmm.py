@@ -211,6 +211,11 @@ class Plan:
         self.U_l = self.info["U_l"]
         self.nnz = self.info["nnz"]
         self.num_rows = self.info["num_rows"]
+        # Largest remote (global) column, the last B row the pack reads: the
+        # partition's remote columns are sorted.
+        arrs = _lib.PartitionArrays()
+        check(lib().cola_partition_get_arrays(partition.handle, ctypes.byref(arrs)))
+        self.max_remote_col = int(arrs.csc_colIdx_r[self.U_r - 1]) if self.U_r > 0 else -1
 
     @property
     def handle(self):
@@ -237,7 +242,7 @@ class Plan:
         if dtype not in DTYPE_BYTES:
             raise ValueError(f"unknown storage dtype {dtype}")
         _check_matrix(packed, "packed", self.U_r, dim, DTYPE_BYTES[dtype])
-        _check_matrix(B_global, "B_global", 1 if self.U_r > 0 else 0, dim, DTYPE_BYTES[dtype])
+        _check_matrix(B_global, "B_global", self.max_remote_col + 1, dim, DTYPE_BYTES[dtype])
         if dtype == FP8_E4M3:
             _check_vector(packed_scales, "packed_scales", self.U_r)
         pack_remote(B_global, self.remote_cols_ptr, packed, dim=dim, dtype=dtype, scales_global=scales_global,
