@@ -13,6 +13,7 @@ from ._lib import (ACCUMULATE, BF16, F16, F32, FP8_E4M3, KAHAN, OVERWRITE, ColaE
 from .partition import (Graph, Partition, bin2vec, load_mtx, nnz_split, process_graph,  # noqa: F401
                         process_graph_ppl, reorder_mtx, serialize, serialize_ppl, stage_cols)
 from .spmm import (DTYPE_BYTES, DTYPE_NAMES, REQUEST_RECORD, Batch, Plan, Server, d_csc_zip, d_csr_zip, dist_spmm_ppl_batch,  # noqa: F401
-                   error_norms, pack_remote, ppl0_batch_SpMM, quantize_rows)
+                   error_norms, pack_remote, ppl0_batch_SpMM, ppl0_batch_SpMM_global, ppl0_plans_clear,
+                   quantize_rows)
 
 __version__ = "0.1.0"

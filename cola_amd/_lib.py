@@ -114,6 +114,10 @@ SIGNATURES = {
     "cola_ppl0_batch_spmm": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                              c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_vp],
     "cola_dist_spmm_ppl_batch": [c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_vp],
+    "cola_ppl0_batch_spmm_global": [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                    c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_vp],
+    "cola_ppl0_plans_clear": [],
+    "cola_warmup": [c_int, c_vp],
     "cola_plan_create": [c_vp, ctypes.POINTER(PlanOpts), ctypes.POINTER(c_vp)],
     "cola_plan_get_info": [c_vp, ctypes.POINTER(PlanInfo)],
     "cola_plan_remote_cols": [c_vp, ctypes.POINTER(c_vp)],

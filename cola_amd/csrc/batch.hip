@@ -276,7 +276,7 @@ static cola_status append_problem(const cola_batch_problem &pr, const cola_batch
       hi = std::max(hi, bb + ((uint64_t)(rows - 1) * (uint64_t)ld + (uint64_t)pr.dim) * eb);
     };
     span(pr.B, p->nodesPerPE, pr.ld_B);
-    span(pr.packed, p->U_r, pr.ld_packed);
+    span(pr.packed, p->remote_rows, pr.ld_packed);
     if (ok && lo != UINT64_MAX && hi - lo < (uint64_t(1) << 36)) abase = lo;
   }
   const PlanBin *bins[3] = {&p->longb, &p->shortb, (flags & COLA_OVERWRITE) ? &p->emptyb : nullptr};

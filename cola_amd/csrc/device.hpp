@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -35,7 +36,9 @@ struct cola_zip_s {
   uint64_t serial = 0;  // unique per created zip (never reused)
   // Plan built from this (local) zip and the remote zip with serial
   // fast_peer by the single-stage wrapper, reused while its arguments match.
-  cola_plan_s *fast_plan = nullptr;
+  // Shared: a call holds its own reference while it enqueues work, so a
+  // concurrent rebuild or cola_zip_invalidate cannot free it under the call.
+  std::shared_ptr<cola_plan_s> fast_plan;
   uint64_t fast_peer = 0;
   int fast_P = -1, fast_peid = -1;
 };
@@ -51,6 +54,10 @@ struct PlanBin {
 
 struct cola_plan_s {
   int32_t num_rows = 0, nodesPerPE = 0, U_r = 0, U_l = 0, max_dim = 0;
+  // Rows a remote entry can address in its table: U_r (packed buffer) or,
+  // for a plan whose remote entries hold global row ids (the reference-
+  // signature launcher reads B_global directly), the largest id + 1.
+  int64_t remote_rows = 0;
   int64_t nnz = 0, num_empty = 0, num_hubs = 0, num_chunks = 0;
   PlanBin shortb, longb;      // every item
   PlanBin shortb_r, longb_r;  // items with remote entries (second phase)

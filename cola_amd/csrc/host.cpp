@@ -348,8 +348,11 @@ cola_status validate_partition(const Partition &p) {
     if (ri[i] < 0 || ri[i] >= M) return bad("csr_rowIdx_l out of [0,M)");
     if (i && ri[i] <= ri[i - 1]) return bad("csr_rowIdx_l not strictly increasing");
   }
+  // Local column c is graph node rank*P + c: below P, and below n on the
+  // last rank, whose slice of B has only M rows.
+  const int64_t lcols = std::min<int64_t>(P, n - (int64_t)p.rank * P);
   for (int32_t c : p.csr_colIdx_l)
-    if (c < 0 || c >= P) return bad("csr_colIdx_l out of [0,P)");
+    if (c < 0 || c >= lcols) return bad("csr_colIdx_l out of [0, min(P, n - rank*P))");
   const auto &cp = p.csc_colPtr_r, &ci = p.csc_colIdx_r;
   if (cp.empty() || cp[0] != 0) return bad("csc_colPtr_r must start at 0");
   if (ci.size() + 1 != cp.size()) return bad("len(csc_colIdx_r) != len(csc_colPtr_r)-1");
@@ -638,6 +641,36 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
   out->panel_chunks = 0;
   if (K > 1) {
     const int64_t P = std::max<int32_t>(1, p.nodesPerPE), U = std::max<int32_t>(1, out->U_r);
+    auto panel_of = [&](int32_t c) {
+      return c >= 0 ? (int)((int64_t)c * K / P) : K + (int)((int64_t)(~c) * K / U);
+    };
+    // Each row becomes one item per panel it touches, so its entries must be
+    // grouped by panel: two items of one row in the same panel launch would
+    // both read-add-write that row of C at once.  Rows are stably sorted by
+    // panel (local panels come before remote ones, so local entries stay
+    // first; the order within a panel is kept).  Rows from the reference's
+    // writer are already sorted by column and are left as they are.
+#pragma omp parallel
+    {
+      std::vector<std::pair<int32_t, float>> row;
+#pragma omp for schedule(dynamic, 1024)
+      for (int64_t i = 0; i < M; ++i) {
+        bool sorted = true;
+        for (int64_t k = cnt[i] + 1; k < cnt[i + 1] && sorted; ++k)
+          sorted = panel_of(out->cols[k - 1]) <= panel_of(out->cols[k]);
+        if (sorted) continue;
+        row.clear();
+        for (int64_t k = cnt[i]; k < cnt[i + 1]; ++k) row.emplace_back(out->cols[k], out->vals[k]);
+        std::stable_sort(row.begin(), row.end(), [&](const std::pair<int32_t, float> &x,
+                                                     const std::pair<int32_t, float> &y) {
+          return panel_of(x.first) < panel_of(y.first);
+        });
+        for (size_t k = 0; k < row.size(); ++k) {
+          out->cols[cnt[i] + k] = row[k].first;
+          out->vals[cnt[i] + k] = row[k].second;
+        }
+      }
+    }
     std::vector<std::vector<WorkItem>> per(2 * K);
     int32_t pch = 0;
     // Same rows, order and hub test as the items above, so panel_hub_ptr
@@ -650,15 +683,9 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
       const int32_t kflag = d > kdeg ? 1 : 0;
       int64_t s = b;
       while (s < e) {
-        const int32_t c = out->cols[s];
-        const int key = c >= 0 ? (int)((int64_t)c * K / P) : K + (int)((int64_t)(~c) * K / U);
+        const int key = panel_of(out->cols[s]);
         int64_t t = s + 1;
-        while (t < e) {
-          const int32_t c2 = out->cols[t];
-          const int k2 = c2 >= 0 ? (int)((int64_t)c2 * K / P) : K + (int)((int64_t)(~c2) * K / U);
-          if (k2 != key) break;
-          ++t;
-        }
+        while (t < e && panel_of(out->cols[t]) == key) ++t;
         if (!is_hub) {
           per[key].push_back({(int32_t)i, (int32_t)s, (int32_t)t, kflag});
         } else {

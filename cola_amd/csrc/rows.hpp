@@ -96,7 +96,9 @@ __device__ __forceinline__ float hub_sum(const float *p, int64_t pstride, int c0
         if (kahan) {
           const float y = x[i] - err;
           const float t = s + y;
-          err = (t - s) - y;
+          // Once the sum is infinite or NaN there is nothing to compensate
+          // (and (t - s) - y would turn an infinite sum into NaN).
+          err = __builtin_isfinite(t) ? (t - s) - y : 0.f;
           s = t;
         } else {
           s += x[i];
@@ -118,6 +120,7 @@ struct Vec<COLA_F32, 4> {
   using Raw = float4;
   static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const float4 *>(p); }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global_f4(a); }
+  static __device__ __forceinline__ Raw zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[4]) {
     x[0] = r.x; x[1] = r.y; x[2] = r.z; x[3] = r.w;
   }
@@ -128,6 +131,7 @@ struct Vec<COLA_F32, 1> {
   using Raw = float;
   static __device__ __forceinline__ Raw load(const T *p) { return *p; }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global<float>(a); }
+  static __device__ __forceinline__ Raw zero() { return 0.f; }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = r; }
 };
 template <int DT>
@@ -136,6 +140,7 @@ struct Vec16 {
   using Raw = uint4;
   static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const uint4 *>(p); }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global_u4(a); }
+  static __device__ __forceinline__ Raw zero() { return make_uint4(0u, 0u, 0u, 0u); }
   static __device__ __forceinline__ float cvt(uint32_t h) {
     if constexpr (DT == COLA_BF16) return bf16_to_f32(h);
     else return f16_to_f32(h);
@@ -159,6 +164,7 @@ struct Vec16x1 {
   using Raw = uint32_t;
   static __device__ __forceinline__ Raw load(const T *p) { return *p; }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global<uint16_t>(a); }
+  static __device__ __forceinline__ Raw zero() { return 0u; }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = Vec16<DT>::cvt(r); }
 };
 template <>
@@ -171,6 +177,7 @@ struct Vec<COLA_FP8_E4M3, 16> {
   using Raw = uint4;
   static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const uint4 *>(p); }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global_u4(a); }
+  static __device__ __forceinline__ Raw zero() { return make_uint4(0u, 0u, 0u, 0u); }
   // gfx950's v_cvt_pk_f32_fp8 decodes OCP E4M3 (checked against the format
   // definition for all 256 codes), two bytes per instruction.
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[16]) {
@@ -192,6 +199,7 @@ struct Vec<COLA_FP8_E4M3, 1> {
   using Raw = uint32_t;
   static __device__ __forceinline__ Raw load(const T *p) { return *p; }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global<uint8_t>(a); }
+  static __device__ __forceinline__ Raw zero() { return 0u; }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = fp8e4m3_to_f32(r); }
 };
 
@@ -401,6 +409,21 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
         // registers and leaves one or two loads in flight instead of U.
         __builtin_amdgcn_sched_barrier(0);
       }
+      // Slots past the item's end hold the last entry's row with value 0;
+      // their rows are replaced by zeros before use, so an Inf or NaN in
+      // that B row cannot turn 0 * x into NaN (only stored entries are
+      // multiplied, as in the reference's loop over [rowPtr[v],
+      // rowPtr[v+1])).  Only a team's last batch can have such slots: the
+      // branch is skipped when no lane of the wave is in one.
+      if (j + gpt * U > n) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int pos = K > 1 ? (u / G) * tsize + (u % G) * gpt + tsub : j + u * gpt + tsub;
+          if (pos >= n)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) raw[u][k] = V::zero();
+        }
+      }
 #pragma unroll
       for (int u = 0; u < U; ++u)
 #pragma unroll
@@ -416,7 +439,7 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
             for (int q = 0; q < EPL; ++q) {
               const float y = fmaf(av[u], x[q], -err[k][q]);
               const float t = acc[k][q] + y;
-              err[k][q] = (t - acc[k][q]) - y;
+              err[k][q] = __builtin_isfinite(t) ? (t - acc[k][q]) - y : 0.f;
               acc[k][q] = t;
             }
           }
@@ -440,7 +463,7 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
           const float t = s1 + s2;
           const float lost = fabsf(s1) >= fabsf(s2) ? (s1 - t) + s2 : (s2 - t) + s1;
           acc[k][q] = t;
-          err[k][q] = err[k][q] + e2 - lost;
+          err[k][q] = __builtin_isfinite(t) ? err[k][q] + e2 - lost : 0.f;
         }
       }
   }

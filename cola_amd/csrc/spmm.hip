@@ -34,6 +34,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <new>
 #include <string>
@@ -185,7 +186,10 @@ __global__ __launch_bounds__(256) void ppl0_kernel(
 #pragma unroll
           for (int k = 0; k < NV; ++k) {
             const int col = (k * G + gl) * EPL;
-            if (col < dim) RowT::widen(RowT::load(rp + col), x[u][k]);
+            // A slot past the row's end (j + u >= cnt) re-reads the last
+            // entry's row; it is replaced by zeros so that an Inf or NaN
+            // there never meets the slot's zero value.
+            if (col < dim && j + u < cnt) RowT::widen(RowT::load(rp + col), x[u][k]);
             else
 #pragma unroll
               for (int q = 0; q < EPL; ++q) x[u][k][q] = 0.f;
@@ -347,10 +351,20 @@ __global__ __launch_bounds__(256) void error_norms_kernel(const float *__restric
 // Handles.
 // ---------------------------------------------------------------------------
 static void free_plan(cola_plan_s *p);
+// Guards the plans the single-stage wrapper caches in zips.
+static std::mutex g_zip_plan_mu;
+
+// Drops the reference-signature launcher's cached plans built from any of
+// these device arrays (their memory is about to be freed and may be reused).
+static void drop_ref_plans_using(const void *a, const void *b, const void *c, const void *d);
 
 static void free_zip(cola_zip_s *z) {
   if (!z) return;
-  free_plan(z->fast_plan);
+  drop_ref_plans_using(z->ptr, z->idx, z->zip, z->val);
+  {
+    std::lock_guard<std::mutex> lock(g_zip_plan_mu);
+    z->fast_plan.reset();
+  }
   (void)hipFree(z->ptr);
   (void)hipFree(z->idx);
   (void)hipFree(z->zip);
@@ -466,7 +480,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
       hi = std::max(hi, b + ((uint64_t)(rows - 1) * (uint64_t)ld + (uint64_t)w) * eb);
     };
     span(c.Bl, p->nodesPerPE, c.ldB);
-    span(c.Bp, p->U_r, c.ldP);
+    span(c.Bp, p->remote_rows, c.ldP);
     if (lo == UINT64_MAX || lo % 16 != 0 || hi - lo >= (uint64_t(1) << 36)) return 0;
     return lo;
   };
@@ -596,10 +610,13 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
 
 }  // namespace
 
-static cola_status plan_from_partition(const cola::Partition &part, const cola_plan_opts &o, cola_plan_s **out);
-// Guards the plans the single-stage wrapper caches in zips.
-static std::mutex g_zip_plan_mu;
-static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int peid, int dim, cola_plan_s **out);
+// direct_remote: remote entries address global row g of B_global (~g)
+// instead of packed row u (~u), for callers that pass B_global as the
+// remote table.
+static cola_status plan_from_partition(const cola::Partition &part, const cola_plan_opts &o, cola_plan_s **out,
+                                       bool direct_remote = false);
+static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int peid, int dim,
+                                 std::shared_ptr<cola_plan_s> *out);
 
 // ---------------------------------------------------------------------------
 // C ABI (device part).
@@ -774,7 +791,9 @@ cola_status cola_dist_spmm_ppl_batch(float *output, const float *B_global, float
   // The same sums as ppl0_batch_SpMM over the virtual rows / columns, run
   // through a plan rebuilt from the two zips on the first call (cached in the
   // local zip): no atomics, deterministic order.
-  cola_plan_s *plan = nullptr;
+  // This call's own reference keeps the plan alive until its launches are
+  // enqueued, whatever other threads do to the zip's cache meanwhile.
+  std::shared_ptr<cola_plan_s> plan;
   cola_status st = zip_pair_plan(csr_zip_l, csc_zip_r, nodesPerPE, peid, dim, &plan);
   if (st) return st;
   if (plan->U_r > 0 && !packed) return fail(COLA_ERR_INVALID, "dist_spmm_ppl_batch: packed is NULL");
@@ -782,15 +801,14 @@ cola_status cola_dist_spmm_ppl_batch(float *output, const float *B_global, float
                         stream);
   if (st) return st;
   const float *input = B_global + (int64_t)peid * nodesPerPE * dim;
-  return cola_spmm_partition(plan, output, dim, input, dim, nullptr, packed, dim, nullptr, dim, COLA_F32,
+  return cola_spmm_partition(plan.get(), output, dim, input, dim, nullptr, packed, dim, nullptr, dim, COLA_F32,
                              COLA_ACCUMULATE, stream);
 }
 
 cola_status cola_zip_invalidate(cola_zip z) {
   if (!z) return fail(COLA_ERR_INVALID, "zip_invalidate: NULL handle");
   std::lock_guard<std::mutex> lock(g_zip_plan_mu);
-  free_plan(z->fast_plan);
-  z->fast_plan = nullptr;
+  z->fast_plan.reset();  // freed once no call in flight holds it
   z->fast_peer = 0;
   return COLA_OK;
 }
@@ -808,15 +826,20 @@ cola_status cola_plan_create(cola_partition part, const cola_plan_opts *opts, co
 
 }  // extern "C"
 
-static cola_status plan_from_partition(const cola::Partition &part, const cola_plan_opts &o, cola_plan_s **out) {
+static cola_status plan_from_partition(const cola::Partition &part, const cola_plan_opts &o, cola_plan_s **out,
+                                       bool direct_remote) {
   cola::HostPlan hp;
   cola_status st = cola::build_plan(part, o, &hp);
   if (st) return st;
+  if (direct_remote)
+    for (int32_t &c : hp.cols)
+      if (c < 0) c = ~hp.remote_cols[~c];
   auto *p = new (std::nothrow) cola_plan_s;
   if (!p) return fail(COLA_ERR_NOMEM, "plan_create: out of host memory");
   p->num_rows = hp.num_rows;
   p->nodesPerPE = hp.nodesPerPE;
   p->U_r = hp.U_r;
+  p->remote_rows = direct_remote ? (hp.remote_cols.empty() ? 0 : (int64_t)hp.remote_cols.back() + 1) : hp.U_r;
   p->U_l = hp.U_l;
   p->max_dim = hp.max_dim;
   p->pstride = (hp.max_dim + 15) / 16 * 16;
@@ -917,15 +940,15 @@ static cola_status download(const T *d, int64_t n, std::vector<T> *h) {
 // virtual rows (columns) that nnz_split cut from one row (column) are
 // consecutive with the same id, so they are merged back; entries keep their
 // order.  Items in decreasing-degree order; the plan's default schedule.
-static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int peid, int dim, cola_plan_s **out) {
+static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int peid, int dim,
+                                 std::shared_ptr<cola_plan_s> *out) {
   std::lock_guard<std::mutex> lock(g_zip_plan_mu);
   if (l->fast_plan && l->fast_peer == r->serial && l->fast_P == P && l->fast_peid == peid &&
       dim <= l->fast_plan->max_dim) {
     *out = l->fast_plan;
     return COLA_OK;
   }
-  free_plan(l->fast_plan);
-  l->fast_plan = nullptr;
+  l->fast_plan.reset();
   std::vector<int32_t> lptr, lidx, lzip, rptr, ridx, rzip;
   std::vector<float> lval, rval;
   const int64_t Lv = l->nptr > 0 ? l->nptr - 1 : 0, Rv = r->nptr > 0 ? r->nptr - 1 : 0;
@@ -963,11 +986,11 @@ static cola_status zip_pair_plan(cola_zip_s *l, const cola_zip_s *r, int P, int 
   o.row_order = 1;
   cola_plan_s *p = nullptr;
   if ((st = plan_from_partition(part, o, &p))) return st;
-  l->fast_plan = p;
+  l->fast_plan = std::shared_ptr<cola_plan_s>(p, free_plan);
   l->fast_peer = r->serial;
   l->fast_P = P;
   l->fast_peid = peid;
-  *out = p;
+  *out = l->fast_plan;
   return COLA_OK;
 }
 
@@ -1232,3 +1255,191 @@ cola_status cola_error_norms(const float *C, int64_t ld_C, const double *R, int6
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Reference-signature launcher (ppl0_batch_SpMM's parameter list): the zips'
+// device arrays and offsets as raw pointers, remote rows read from B_global
+// at their global row (input is B_global + peid * nodesPerPE * dim).
+// ---------------------------------------------------------------------------
+namespace {
+
+// Arguments that determine the work of one call; a plan is cached per key.
+struct RefKey {
+  const void *arr[8];
+  int llb, rlb, lnum, rnum, P, peid;
+  bool operator<(const RefKey &o) const {
+    for (int i = 0; i < 8; ++i)
+      if (arr[i] != o.arr[i]) return arr[i] < o.arr[i];
+    const int a[6] = {llb, rlb, lnum, rnum, P, peid}, b[6] = {o.llb, o.rlb, o.lnum, o.rnum, o.P, o.peid};
+    return std::lexicographical_compare(a, a + 6, b, b + 6);
+  }
+};
+std::mutex g_ref_mu;
+std::map<RefKey, std::shared_ptr<cola_plan_s>> g_ref_plans;
+
+template <typename T>
+cola_status download_range(const T *d, int64_t first, int64_t n, std::vector<T> *h) {
+  h->assign((size_t)std::max<int64_t>(n, 0), T());
+  if (n <= 0) return COLA_OK;
+  return hip_check(hipMemcpy(h->data(), d + first, (size_t)n * sizeof(T), hipMemcpyDeviceToHost),
+                   "hipMemcpy reference arrays");
+}
+
+// Rebuilds the partition that virtual rows [llb, llb+lnum) of the local zip
+// CSR and virtual columns [rlb, rlb+rnum) of the remote zip CSC describe.
+// Entries of one output row (column) are gathered in the order the virtual
+// rows (columns) list them, whatever the order of those.
+cola_status ref_partition(const RefKey &k, const int32_t *rowPtr_l, const int32_t *colIdx_l,
+                          const int32_t *rowIdx_l, const float *val_l, const int32_t *colPtr_r,
+                          const int32_t *rowIdx_r, const int32_t *colIdx_r, const float *val_r,
+                          cola::Partition *part) {
+  const int P = k.P;
+  std::vector<int32_t> lptr, lrow, lcol, rptr, rcol, rrow;
+  std::vector<float> lval, rval;
+  cola_status st;
+  if ((st = download_range(rowPtr_l, k.llb, k.lnum > 0 ? k.lnum + 1 : 0, &lptr)) ||
+      (st = download_range(rowIdx_l, k.llb, k.lnum, &lrow)) ||
+      (st = download_range(colPtr_r, k.rlb, k.rnum > 0 ? k.rnum + 1 : 0, &rptr)) ||
+      (st = download_range(colIdx_r, k.rlb, k.rnum, &rcol)))
+    return st;
+  for (int v = 0; v < k.lnum; ++v)
+    if (lptr[v + 1] < lptr[v]) return fail(COLA_ERR_INVALID, "ppl0_batch_spmm_global: csr_rowPtr_l decreasing");
+  for (int v = 0; v < k.rnum; ++v)
+    if (rptr[v + 1] < rptr[v]) return fail(COLA_ERR_INVALID, "ppl0_batch_spmm_global: csc_colPtr_r decreasing");
+  const int64_t l0 = k.lnum > 0 ? lptr[0] : 0, l1 = k.lnum > 0 ? lptr[k.lnum] : 0;
+  const int64_t r0 = k.rnum > 0 ? rptr[0] : 0, r1 = k.rnum > 0 ? rptr[k.rnum] : 0;
+  if ((st = download_range(colIdx_l, l0, l1 - l0, &lcol)) || (st = download_range(val_l, l0, l1 - l0, &lval)) ||
+      (st = download_range(rowIdx_r, r0, r1 - r0, &rrow)) || (st = download_range(val_r, r0, r1 - r0, &rval)))
+    return st;
+  for (int32_t r : lrow)
+    if (r < 0 || r >= P) return fail(COLA_ERR_INVALID, "ppl0_batch_spmm_global: csr_rowIdx_zip_l out of [0,P)");
+  // Local part: output rows in increasing order, each with its entries in
+  // virtual-row order (a stable counting sort of the virtual rows).
+  std::vector<int64_t> rcnt((size_t)P + 1, 0);
+  for (int v = 0; v < k.lnum; ++v) rcnt[lrow[v] + 1] += lptr[v + 1] - lptr[v];
+  for (int i = 0; i < P; ++i) rcnt[i + 1] += rcnt[i];
+  part->csr_colIdx_l.resize(lcol.size());
+  part->csr_val_l.resize(lcol.size());
+  {
+    std::vector<int64_t> cur(rcnt.begin(), rcnt.end() - 1);
+    for (int v = 0; v < k.lnum; ++v)
+      for (int64_t q = lptr[v]; q < lptr[v + 1]; ++q) {
+        const int64_t w = cur[lrow[v]]++;
+        part->csr_colIdx_l[w] = lcol[q - l0];
+        part->csr_val_l[w] = lval[q - l0];
+      }
+  }
+  part->csr_rowPtr_l.assign(1, 0);
+  for (int i = 0; i < P; ++i)
+    if (rcnt[i + 1] > rcnt[i]) {
+      part->csr_rowIdx_l.push_back(i);
+      part->csr_rowPtr_l.push_back((int32_t)rcnt[i + 1]);
+    }
+  // Remote part: distinct global columns in increasing order, each with its
+  // entries in virtual-column order.
+  std::vector<int32_t> order(k.rnum);
+  for (int v = 0; v < k.rnum; ++v) order[v] = v;
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return rcol[a] < rcol[b]; });
+  part->csc_colPtr_r.assign(1, 0);
+  for (int oi = 0; oi < k.rnum; ++oi) {
+    const int v = order[oi];
+    if (rptr[v + 1] == rptr[v]) continue;
+    if (part->csc_colIdx_r.empty() || part->csc_colIdx_r.back() != rcol[v]) {
+      part->csc_colIdx_r.push_back(rcol[v]);
+      part->csc_colPtr_r.push_back(part->csc_colPtr_r.back());
+    }
+    for (int64_t q = rptr[v]; q < rptr[v + 1]; ++q) {
+      part->csc_rowIdx_r.push_back(rrow[q - r0]);
+      part->csc_val_r.push_back(rval[q - r0]);
+    }
+    part->csc_colPtr_r.back() = (int32_t)part->csc_rowIdx_r.size();
+  }
+  part->nodesPerPE = P;
+  part->rank = k.peid;
+  part->num_rows = P;
+  const int64_t max_col = part->csc_colIdx_r.empty() ? 0 : std::max<int64_t>(0, part->csc_colIdx_r.back());
+  part->nGPU = (int32_t)std::max<int64_t>(k.peid + 1, max_col / P + 1);
+  part->num_nodes = (int64_t)part->nGPU * P;
+  return cola::validate_partition(*part);
+}
+
+}  // namespace
+
+static void drop_ref_plans_using(const void *a, const void *b, const void *c, const void *d) {
+  std::lock_guard<std::mutex> lock(g_ref_mu);
+  for (auto it = g_ref_plans.begin(); it != g_ref_plans.end();) {
+    bool uses = false;
+    for (const void *x : it->first.arr) uses = uses || (x && (x == a || x == b || x == c || x == d));
+    it = uses ? g_ref_plans.erase(it) : std::next(it);
+  }
+}
+
+extern "C" {
+
+cola_status cola_ppl0_batch_spmm_global(float *output, const float *input, const int32_t *csr_rowPtr_l,
+                                        const int32_t *csr_colIdx_l, const int32_t *csr_rowIdx_zip_l,
+                                        const float *csr_val_l, const int32_t *csc_colPtr_r,
+                                        const int32_t *csc_rowIdx_r, const int32_t *csc_colIdx_zip_r,
+                                        const float *csc_val_r, int local_row_lb, int remote_col_lb,
+                                        int local_row_num, int remote_col_num, int dim, int nodesPerPE, int peid,
+                                        int literal, void *stream) {
+  if (local_row_num < 0 || remote_col_num < 0 || local_row_lb < 0 || remote_col_lb < 0 || nodesPerPE < 1 ||
+      peid < 0 || dim < 1)
+    return fail(COLA_ERR_INVALID, "ppl0_batch_spmm_global: negative range or bad sizes");
+  if (local_row_num + (int64_t)remote_col_num == 0) return COLA_OK;  // cuh:232
+  if (!output || !input) return fail(COLA_ERR_INVALID, "ppl0_batch_spmm_global: output or input is NULL");
+  if (local_row_num > 0 && (!csr_rowPtr_l || !csr_colIdx_l || !csr_rowIdx_zip_l || !csr_val_l))
+    return fail(COLA_ERR_INVALID, "ppl0_batch_spmm_global: NULL local array");
+  if (remote_col_num > 0 && (!csc_colPtr_r || !csc_rowIdx_r || !csc_colIdx_zip_r || !csc_val_r))
+    return fail(COLA_ERR_INVALID, "ppl0_batch_spmm_global: NULL remote array");
+  // Remote global row g lives at input + (g - peid*P)*dim: B_global's base.
+  const float *B_global = input - (int64_t)peid * nodesPerPE * dim;
+  if (literal)
+    return cola_ppl0_batch_spmm(output, input, B_global, csr_rowPtr_l, csr_colIdx_l, csr_rowIdx_zip_l, csr_val_l,
+                                csc_colPtr_r, csc_rowIdx_r, csc_colIdx_zip_r, csc_val_r, local_row_lb,
+                                remote_col_lb, local_row_num, remote_col_num, dim, nodesPerPE, peid, stream);
+  RefKey key{{csr_rowPtr_l, csr_colIdx_l, csr_rowIdx_zip_l, csr_val_l, csc_colPtr_r, csc_rowIdx_r,
+              csc_colIdx_zip_r, csc_val_r},
+             local_row_lb, remote_col_lb, local_row_num, remote_col_num, nodesPerPE, peid};
+  std::shared_ptr<cola_plan_s> plan;
+  {
+    std::lock_guard<std::mutex> lock(g_ref_mu);
+    auto it = g_ref_plans.find(key);
+    if (it != g_ref_plans.end() && dim <= it->second->max_dim) plan = it->second;
+  }
+  if (!plan) {
+    cola::Partition part;
+    cola_status st = ref_partition(key, csr_rowPtr_l, csr_colIdx_l, csr_rowIdx_zip_l, csr_val_l, csc_colPtr_r,
+                                   csc_rowIdx_r, csc_colIdx_zip_r, csc_val_r, &part);
+    if (st) return st;
+    cola_plan_opts o{};
+    o.max_dim = std::max(dim, 256);
+    o.row_order = 1;
+    cola_plan_s *p = nullptr;
+    if ((st = plan_from_partition(part, o, &p, true))) return st;
+    plan = std::shared_ptr<cola_plan_s>(p, free_plan);
+    std::lock_guard<std::mutex> lock(g_ref_mu);
+    g_ref_plans[key] = plan;
+  }
+  return cola_spmm_partition(plan.get(), output, dim, input, dim, nullptr, B_global, dim, nullptr, dim, COLA_F32,
+                             COLA_ACCUMULATE, stream);
+}
+
+cola_status cola_ppl0_plans_clear(void) {
+  std::lock_guard<std::mutex> lock(g_ref_mu);
+  g_ref_plans.clear();  // each plan is freed once no call in flight holds it
+  return COLA_OK;
+}
+
+}  // extern "C"
+
+namespace {
+__global__ void warmup_kernel() {}
+}  // namespace
+
+extern "C" cola_status cola_warmup(int launches, void *stream) {
+  if (launches < 0) return fail(COLA_ERR_INVALID, "warmup: negative count");
+  for (int i = 0; i < launches; ++i)
+    hipLaunchKernelGGL(warmup_kernel, dim3(1024), dim3(256), 0, (hipStream_t)stream);
+  return hip_check(hipGetLastError(), "warmup launch");
+}

@@ -179,6 +179,35 @@ def ppl0_batch_SpMM(output, input, packed, csr_zip_l, csc_zip_r, local_row_lb, r
         int(dim), int(nodesPerPE), int(peid), _stream(stream)))
 
 
+def ppl0_batch_SpMM_global(output, input, csr_zip_l, csc_zip_r, local_row_lb, remote_col_lb, local_row_num,
+                           remote_col_num, dim, nodesPerPE, peid, literal=False, stream=None):
+    """ppl0_batch_SpMM with the reference's parameter list (cuh:202-221):
+    remote global row g is read at input + (g - peid*nodesPerPE)*dim, so
+    `input` must be a row slice of one B_global holding every partition (a
+    view B_global[peid*P:] is).  literal=False: cached plan (no atomics);
+    literal=True: the virtual-row kernel with FP32 atomics."""
+    if local_row_num + remote_col_num > 0:
+        _check_matrix(output, "output", max(csr_zip_l.max_id, csc_zip_r.max_idx) + 1, dim, 4)
+        _check_matrix(input, "input", csr_zip_l.max_idx + 1 if local_row_num > 0 else 0, dim, 4)
+        if input.stride(0) != dim or output.stride(0) != dim:
+            raise ValueError("input and output must have rows of exactly dim elements")
+        base = input.data_ptr() - peid * nodesPerPE * dim * 4
+        if remote_col_num > 0 and (base < input.untyped_storage().data_ptr() or
+                                   base + (csc_zip_r.max_id + 1) * dim * 4 >
+                                   input.untyped_storage().data_ptr() + input.untyped_storage().nbytes()):
+            raise ValueError("input must be the partition's slice of a B_global holding every remote row")
+    check(lib().cola_ppl0_batch_spmm_global(
+        _dptr(output), _dptr(input), csr_zip_l._ptr, csr_zip_l._idx, csr_zip_l._zip, csr_zip_l._val,
+        csc_zip_r._ptr, csc_zip_r._idx, csc_zip_r._zip, csc_zip_r._val,
+        int(local_row_lb), int(remote_col_lb), int(local_row_num), int(remote_col_num),
+        int(dim), int(nodesPerPE), int(peid), 1 if literal else 0, _stream(stream)))
+
+
+def ppl0_plans_clear():
+    """Drops the plans ppl0_batch_SpMM_global cached."""
+    check(lib().cola_ppl0_plans_clear())
+
+
 def dist_spmm_ppl_batch(output, B_global, packed, csr_zip_l, csc_zip_r, dim, nodesPerPE, peid, stream=None):
     """Single-stage distSpMM_ppl_batch_wrapper: pack the unique remote rows, then
     accumulate every virtual local row and remote column into output."""
@@ -253,7 +282,8 @@ class Plan:
         dim = dim if dim is not None else C.shape[1]
         self._check_call(C, B_local, packed, dim, dtype, scales_local, packed_scales)
         ldp = packed.stride(0) if packed is not None and packed.dim() == 2 else dim
-        check(lib().cola_spmm_partition(self._h, _dptr(C), C.stride(0), _dptr(B_local), B_local.stride(0),
+        ldb = B_local.stride(0) if B_local is not None else dim
+        check(lib().cola_spmm_partition(self._h, _dptr(C), C.stride(0), _dptr(B_local), ldb,
                                         _dptr(scales_local), _dptr(packed), ldp, _dptr(packed_scales),
                                         int(dim), int(dtype), int(flags), _stream(stream)))
 
@@ -281,7 +311,8 @@ class Plan:
         ldp = packed.stride(0) if packed is not None and packed.dim() == 2 else dim
         best = _lib.PlanTuning()
         ms = ctypes.c_float()
-        check(lib().cola_plan_autotune(self._h, _dptr(C), C.stride(0), _dptr(B_local), B_local.stride(0),
+        ldb = B_local.stride(0) if B_local is not None else dim
+        check(lib().cola_plan_autotune(self._h, _dptr(C), C.stride(0), _dptr(B_local), ldb,
                                        _dptr(scales_local), _dptr(packed), ldp, _dptr(packed_scales), int(dim),
                                        int(dtype), int(flags), int(iters), _stream(stream), ctypes.byref(best),
                                        ctypes.byref(ms)))
@@ -364,6 +395,7 @@ class Server:
         check(lib().cola_server_create(ctypes.byref(o), ctypes.byref(h)))
         self._h = h
         self._keep = []
+        self._out_shape = {}
         self.dtype = dtype
 
     def __del__(self):
@@ -380,6 +412,8 @@ class Server:
         check(lib().cola_server_add_resident(self._h, plan.handle, _dptr(B), B.stride(0), _dptr(scales), int(dim),
                                              ctypes.byref(rid)))
         self._keep.append((plan, B, scales))
+        # Rows and dim of each resident's output, for the checks of run_trace.
+        self._out_shape[rid.value] = (plan.num_rows, int(dim))
         return rid.value
 
     def set_tuning(self, dim, col_tile=0, rpw_short=0, rpw_long=0):
@@ -394,6 +428,19 @@ class Server:
         recs = np.zeros(n, REQUEST_RECORD)
         outp = None
         if outputs is not None:
+            if len(outputs) != n:
+                raise ValueError(f"outputs has {len(outputs)} entries; the trace has {n} requests")
+            for i, o in enumerate(outputs):
+                if o is None:
+                    continue
+                rid = int(res[i])
+                if rid not in self._out_shape:
+                    raise ValueError(f"request {i}: unknown resident id {rid}")
+                rows, dim = self._out_shape[rid]
+                # The server writes rows x dim floats with a row stride of dim.
+                _check_matrix(o, f"outputs[{i}]", rows, dim, 4)
+                if rows > 0 and o.stride(0) != dim:
+                    raise ValueError(f"outputs[{i}] has row stride {o.stride(0)}; the server writes rows of {dim}")
             outp = (ctypes.c_void_p * n)(*[(o.data_ptr() if o is not None else None) for o in outputs])
         wall = ctypes.c_double()
         peak = ctypes.c_int64()

@@ -282,12 +282,52 @@ cola_status cola_ppl0_batch_spmm(
  * literal virtual-row kernel.  The plan copies the zips' arrays at that
  * first call: after writing new values or indices into a zip's device arrays,
  * call cola_zip_invalidate on the local zip (or create new zips).  Calls on
- * one zip from several threads are serialised while the plan is built;
+ * one zip from several threads are serialised while the plan is built, and
+ * each call holds its own reference to the plan until its launches are
+ * enqueued (a concurrent rebuild or invalidate frees it only afterwards);
  * concurrent launches race on its hub workspace. */
 cola_status cola_dist_spmm_ppl_batch(float *output, const float *B_global,
                                      float *packed, cola_zip csr_zip_l,
                                      cola_zip csc_zip_r, int dim,
                                      int nodesPerPE, int peid, void *stream);
+
+/* ppl0_batch_SpMM<CF,is_residue> with the reference's own parameter list
+ * (dist_spmm_ppl_batch.cuh:202-221; call sites cuh:841-846 and cuh:799-804):
+ * the zips' device arrays (d_csr_zip / d_csc_zip members), the virtual-row
+ * range [local_row_lb, +local_row_num) of the local zip CSR and the virtual-
+ * column range [remote_col_lb, +remote_col_num) of the remote zip CSC, whose
+ * csc_colIdx_zip_r holds global column ids.  `input` is this partition's
+ * slice of a B_global that holds every partition back to back, so remote
+ * global row g is read at input + (g - peid*nodesPerPE)*dim (on one GPU the
+ * NVSHMEM get of cuh:120-125 becomes a plain load; no pack is needed).
+ * output (nodesPerPE x dim, ld = dim) is accumulated into, as there.
+ * literal = 0 (the default of the C++ header): the sums run through a
+ * degree-binned plan rebuilt from the arrays on the first call with these
+ * pointers and ranges, then cached (no atomics, deterministic order);
+ * literal = 1: the virtual-row kernel with FP32 atomics (cuh:76-200).
+ * Any dim >= 1 is computed (the reference computes only dim <= 64 and
+ * 97..128).  The cache is keyed by the eight array pointers and the six
+ * integer arguments; destroying a zip (cola_zip_destroy, the sparse_data.h
+ * destructors) drops the plans built from its arrays.  After rewriting the
+ * arrays behind cached pointers, or freeing arrays that no zip owns, call
+ * cola_ppl0_plans_clear.  Remote columns must lie outside the partition's
+ * own rows [peid*P, (peid+1)*P) (prep_torch_par.py:40-79 puts those in the
+ * local CSR). */
+cola_status cola_ppl0_batch_spmm_global(
+    float *output, const float *input, const int32_t *csr_rowPtr_l,
+    const int32_t *csr_colIdx_l, const int32_t *csr_rowIdx_zip_l,
+    const float *csr_val_l, const int32_t *csc_colPtr_r,
+    const int32_t *csc_rowIdx_r, const int32_t *csc_colIdx_zip_r,
+    const float *csc_val_r, int local_row_lb, int remote_col_lb,
+    int local_row_num, int remote_col_num, int dim, int nodesPerPE, int peid,
+    int literal, void *stream);
+/* Drops every plan cola_ppl0_batch_spmm_global cached; each is freed once
+ * no call in flight holds it.  No counterpart in the reference (its launcher
+ * keeps no state). */
+cola_status cola_ppl0_plans_clear(void);
+/* `launches` launches of an empty 1024 x 256 kernel (warmup_kernel,
+ * utils.cuh:24-30, launched 50 times by the harness at cuh:966-969). */
+cola_status cola_warmup(int launches, void *stream);
 /* Drops the plan cola_dist_spmm_ppl_batch cached in a (local) zip; the next
  * call rebuilds it from the zips' current arrays.  No counterpart in the
  * reference (whose wrapper reads the zips on every call). */

@@ -57,3 +57,13 @@ def test_cpp_drop_in_driver_compiles_and_links(tmp_path):
     from tests.cpp_driver import build_driver
     exe = build_driver(tmp_path)
     assert os.path.exists(exe)
+
+
+def test_cpp_reference_driver_compiles_and_links(tmp_path):
+    """A driver whose load, struct and call lines are those of the reference's
+    src/distSpMM_ppl_batch.cu (bin2vec, nnz_split, d_csc_zip / d_csr_zip from
+    std::vectors, run_distSpMM_ppl_batch<true, ...>) builds against
+    include/dist_spmm_ppl_batch.h and the library."""
+    from tests.cpp_driver import build_driver
+    exe = build_driver(tmp_path, "reference_driver")
+    assert os.path.exists(exe)

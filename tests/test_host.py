@@ -231,3 +231,17 @@ def test_graph_digest_is_stable():
     rp, ci, va = g.arrays()
     h = hashlib.sha256(ci[:1000].tobytes()).hexdigest()
     assert h == hashlib.sha256(cola.Graph.rmat(16, 16, seed=1).arrays()[1][:1000].tobytes()).hexdigest()
+
+
+def test_partition_rejects_local_columns_past_the_last_node():
+    """On the last rank (num_rows < nodesPerPE) a local column c is node
+    rank*P + c, which must exist: columns in [M, P) are rejected."""
+    g = cola.Graph.rmat(10, 8, seed=1)  # n = 1024, 3 ranks: P = 342, last M = 340
+    a = g.partition(3, 2).arrays()
+    info = g.partition(3, 2).info()
+    assert info["num_rows"] < info["nodesPerPE"]
+    bad = a["csr_colIdx_l"].copy()
+    bad[0] = info["num_rows"]  # node 1024 does not exist
+    with pytest.raises(ColaError):
+        cola.Partition.from_arrays(g.n, 3, 2, a["csr_rowPtr_l"], bad, a["csr_rowIdx_l"],
+                                   a["csc_colPtr_r"], a["csc_rowIdx_r"], a["csc_colIdx_r"])

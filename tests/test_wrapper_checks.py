@@ -57,3 +57,26 @@ def test_vector_check():
     for bad in (FakeTensor((4,)), FakeTensor((5,), elem=2), FakeTensor((5,), cuda=False), None):
         with pytest.raises(ValueError):
             spmm._check_vector(bad, "scales", 5)
+
+
+def _server_stub(shapes):
+    """A Server whose residents have these (rows, dim) outputs, without a
+    library handle: run_trace's checks run before any library call."""
+    s = object.__new__(spmm.Server)
+    s._out_shape = dict(shapes)
+    s._h = None
+    return s
+
+
+def test_server_run_trace_checks_outputs():
+    s = _server_stub({0: (10, 8), 1: (4, 16)})
+    arr, res = [0.0, 1.0], [0, 1]
+    for outs in ([FakeTensor((9, 8)), None],                  # too few rows
+                 [FakeTensor((10, 8)), FakeTensor((4, 8))],   # too few columns
+                 [FakeTensor((10, 8), strides=(16, 1)), None],  # row stride != dim
+                 [FakeTensor((10, 8), elem=2), None],         # element size
+                 [FakeTensor((10, 8))]):                      # one output per request
+        with pytest.raises(ValueError):
+            s.run_trace(arr, res, outs)
+    with pytest.raises(ValueError):
+        _server_stub({0: (10, 8)}).run_trace([0.0], [3], [FakeTensor((10, 8))])  # unknown resident
