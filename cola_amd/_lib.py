@@ -128,6 +128,8 @@ SIGNATURES = {
     "cola_plan_destroy": [c_vp],
     "cola_spmm_partition": [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_int, c_int,
                             c_int, c_vp],
+    "cola_spmm_partition_noalloc": [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_int, c_int,
+                                    c_int, c_vp],
     "cola_batch_create": [ctypes.POINTER(BatchProblem), c_int, c_int, c_int, ctypes.POINTER(c_vp)],
     "cola_batch_set_tuning": [c_vp, c_int, ctypes.POINTER(BatchTuning), c_int],
     "cola_batch_get_tuning": [c_vp, c_int, ctypes.POINTER(BatchTuning), ctypes.POINTER(c_int)],

@@ -439,15 +439,19 @@ struct SpmmCall {
   int flags;
 };
 
-// Validates a call and runs it with the given tuning.
+// Validates a call and runs it with the given tuning.  ws / ws_stride: the
+// hub-chunk workspace (one row of ws_stride floats per chunk); NULL = the
+// plan's own.
 cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tuning &t,
-                     hipStream_t s) {
+                     hipStream_t s, float *ws = nullptr, int64_t ws_stride = 0) {
+  float *const partials = ws ? ws : p->partials;
+  const int64_t pstride = ws ? ws_stride : p->pstride;
   const size_t eb = dtype_bytes(c.dtype);
   if (!eb) return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: unknown dtype");
   if (c.flags & ~(COLA_OVERWRITE | COLA_KAHAN)) return fail(COLA_ERR_INVALID, "spmm_partition: unknown flag");
   if (c.dim < 1 || c.ldC < c.dim || c.ldB < c.dim || (p->U_r > 0 && c.ldP < c.dim))
     return fail(COLA_ERR_INVALID, "spmm_partition: bad dim or leading dimension");
-  if (c.dim > p->max_dim && (p->num_chunks > 0 || p->panel_chunks > 0))
+  if (!ws && c.dim > p->max_dim && (p->num_chunks > 0 || p->panel_chunks > 0))
     return fail(COLA_ERR_INVALID, "spmm_partition: dim exceeds the plan's max_dim");
   if (p->num_rows > 0 && !c.C) return fail(COLA_ERR_INVALID, "spmm_partition: C is NULL");
   if (p->U_l > 0 && !c.Bl) return fail(COLA_ERR_INVALID, "spmm_partition: B_local is NULL");
@@ -522,8 +526,8 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
         a.sp = c.sp;
         a.C = c.C + c0;
         a.ldC = c.ldC;
-        a.partials = p->partials ? p->partials + c0 : nullptr;
-        a.pstride = p->pstride;
+        a.partials = partials ? partials + c0 : nullptr;
+        a.pstride = pstride;
         a.dim = w;
         a.accumulate = 1;
         a.kahan = kahan;
@@ -542,7 +546,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
       const int64_t threads = p->num_hubs * (int64_t)c.dim;
       const int64_t blocks = (threads + 255) / 256;
       hipLaunchKernelGGL(hub_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p->hub_rows, p->panel_hub_ptr,
-                         (int)p->num_hubs, p->partials, p->pstride, c.C, c.ldC, c.dim, 1, kahan);
+                         (int)p->num_hubs, partials, pstride, c.C, c.ldC, c.dim, 1, kahan);
     }
     return hip_check(hipGetLastError(), "spmm_partition launch");
   }
@@ -579,8 +583,8 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
         a.sp = c.sp;
         a.C = c.C + c0;
         a.ldC = c.ldC;
-        a.partials = p->partials ? p->partials + c0 : nullptr;
-        a.pstride = p->pstride;
+        a.partials = partials ? partials + c0 : nullptr;
+        a.pstride = pstride;
         a.dim = w;
         a.accumulate = accumulate;
         a.kahan = kahan;
@@ -602,7 +606,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
     const int64_t threads = p->num_hubs * (int64_t)c.dim;
     const int64_t blocks = (threads + 255) / 256;
     hipLaunchKernelGGL(hub_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p->hub_rows,
-                       p->hub_ptr, (int)p->num_hubs, p->partials, p->pstride, c.C, c.ldC, c.dim,
+                       p->hub_ptr, (int)p->num_hubs, partials, pstride, c.C, c.ldC, c.dim,
                        accumulate, kahan);
   }
   return hip_check(hipGetLastError(), "spmm_partition launch");
@@ -1049,6 +1053,69 @@ cola_status cola_spmm_partition(cola_plan p, float *C, int64_t ld_C, const void 
   if (!p) return fail(COLA_ERR_INVALID, "spmm_partition: NULL plan");
   SpmmCall c{C, ld_C, B_local, ld_B, scales_local, packed, ld_packed, packed_scales, dim, (int)dtype, flags};
   return run_plan(p, c, p->tune, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free form of cola_spmm_partition: the hub-chunk partial sums go
+// to `scratch`, a fixed device array every GPU holds for the life of the
+// process, instead of the plan's workspace.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int64_t kScratchBytes = COLA_NOALLOC_SCRATCH_BYTES;
+__device__ __align__(256) float scratch[kScratchBytes / sizeof(float)];
+
+std::mutex g_scratch_mu;
+float *g_scratch_dev[64] = {};  // scratch's device address, per device
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each device holds its own copy of the array).  Host code never
+// uses the array's name as an address: only this pointer.
+cola_status scratch_address(float **out) {
+  int dev = -1;
+  cola_status st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (st) return st;
+  if (dev < 0 || dev >= 64) return fail(COLA_ERR_UNSUPPORTED, "spmm_partition_noalloc: device id above 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    if ((st = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")))
+      return st;
+    if (!ptr) return fail(COLA_ERR_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_dev[dev] = static_cast<float *>(ptr);
+  }
+  *out = g_scratch_dev[dev];
+  return COLA_OK;
+}
+}  // namespace
+
+extern "C" {
+
+cola_status cola_spmm_partition_noalloc(cola_plan p, float *C, int64_t ld_C, const void *B_local, int64_t ld_B,
+                                        const float *scales_local, const void *packed, int64_t ld_packed,
+                                        const float *packed_scales, int dim, cola_dtype dtype, int flags,
+                                        void *stream) {
+  if (!p) return fail(COLA_ERR_INVALID, "spmm_partition_noalloc: NULL plan");
+  if (dim < 1) return fail(COLA_ERR_INVALID, "spmm_partition_noalloc: bad dim");
+  // One row of the workspace per hub chunk (of whichever schedule runs),
+  // rounded to 16 floats as the plan's own workspace is.
+  const int64_t stride = ((int64_t)dim + 15) / 16 * 16;
+  const int64_t chunks = (p->tune.panels && p->panels > 1) ? p->panel_chunks : p->num_chunks;
+  const int64_t need = chunks * stride * (int64_t)sizeof(float);
+  if (need > kScratchBytes)
+    return fail(COLA_ERR_INVALID, "spmm_partition_noalloc: the plan's hub rows need " + std::to_string(need) +
+                                      " bytes of workspace at dim " + std::to_string(dim) + "; the fixed array holds " +
+                                      std::to_string(kScratchBytes) + " (use cola_spmm_partition)");
+  float *ws = nullptr;
+  if (chunks > 0) {
+    cola_status st = scratch_address(&ws);
+    if (st) return st;
+  }
+  SpmmCall c{C, ld_C, B_local, ld_B, scales_local, packed, ld_packed, packed_scales, dim, (int)dtype, flags};
+  // Every partial is written before it is read (chunk items store, the hub
+  // reduction then loads), so nothing in the array needs clearing.
+  return run_plan(p, c, p->tune, (hipStream_t)stream, ws, stride);
 }
 
 cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *B_local,

@@ -437,6 +437,26 @@ cola_status cola_spmm_partition(cola_plan plan, float *C, int64_t ld_C,
                                 int dim, cola_dtype dtype, int flags,
                                 void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): cola_spmm_partition with the hub rows' chunk partial sums in a
+ * fixed device array of COLA_NOALLOC_SCRATCH_BYTES that the library holds on
+ * every GPU it runs on until the process ends, instead of the plan's own
+ * workspace.  Same kernels, same summation order, so the result is bitwise
+ * that of cola_spmm_partition.  Calls no hipMalloc / hipFree or allocating
+ * routine.  A plan whose hub chunks need more than the array at this dim
+ * (chunks x round_up(dim, 16) x 4 bytes) is rejected with COLA_ERR_INVALID.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (different streams or threads) race. */
+#define COLA_NOALLOC_SCRATCH_BYTES (16 << 20)
+cola_status cola_spmm_partition_noalloc(cola_plan plan, float *C, int64_t ld_C,
+                                        const void *B_local, int64_t ld_B,
+                                        const float *scales_local,
+                                        const void *packed, int64_t ld_packed,
+                                        const float *packed_scales, int dim,
+                                        cola_dtype dtype, int flags,
+                                        void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batched form: many independent SpMMs in one persistent launch.            */
 /* ------------------------------------------------------------------------ */
