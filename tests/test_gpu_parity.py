@@ -667,18 +667,26 @@ def sampled_rows_csr(rp, ci, va, rows):
     return rp2, inv.astype(np.int32), np.asarray(va, np.float32)[idx], cols
 
 
+@pytest.mark.parametrize("dim", [64, 128, 256])
 @pytest.mark.parametrize("dtype", [cola.BF16, cola.F16, cola.FP8_E4M3], ids=["bf16", "f16", "fp8"])
-def test_full_size_precision_forms(full_size, dtype):
+def test_full_size_precision_forms(full_size, dtype, dim):
     """B stored in BF16 / FP16 / FP8-e4m3 (+ row scales) on the 2^22-node
-    partition, FP32 accumulation with Kahan on rows above 1024 entries: on
-    527 sampled rows the result equals the FP64 product of the dequantised B
-    to FP32 accuracy, and is within the storage format's error of the FP64
-    product of the FP32 B."""
-    g, part, B = full_size["g"], full_size["part"], full_size["B"]
+    partition at each feature dim of the precision configuration (64, 128,
+    256), FP32 accumulation with Kahan on rows above 1024 entries: on 527
+    sampled rows the result equals the FP64 product of the dequantised B to
+    FP32 accuracy (1e-5), and is within the storage format's error of the
+    FP64 product of the FP32 B."""
+    g, part = full_size["g"], full_size["part"]
     rp, ci, va = g.arrays()
     info = part.info()
-    P, M, dim = info["nodesPerPE"], info["num_rows"], 128
-    print(f"case full size precision order={full_size['order']} dtype={cola.DTYPE_NAMES[dtype]}", flush=True)
+    P, M = info["nodesPerPE"], info["num_rows"]
+    if dim == 128:
+        B = full_size["B"]
+    else:
+        gen = torch.Generator(device="cuda").manual_seed(50 + dim)
+        B = torch.rand((P * 8, dim), generator=gen, device="cuda", dtype=torch.float32) * 2 - 1
+    print(f"case full size precision order={full_size['order']} dtype={cola.DTYPE_NAMES[dtype]} dim={dim}",
+          flush=True)
     Bq, sc = cola.quantize_rows(B, dtype)
     plan = cola.Plan(part, row_order=1)
     tdt = {1: torch.uint8, 2: torch.uint16}[cola.DTYPE_BYTES[dtype]]
