@@ -276,7 +276,10 @@ def cpu_section(orc, log, rp, ci, va, lb, M, Bh, N, U_total, seconds):
     frac_rows = done_rows / M
     cb = 4 * (done_rows + 1) + done_nnz * 8 + int(U_total * frac_rows) * N * 4 + done_rows * N * 4
     cpu = {"value": cb / t_cpu / 1e9, "unit": "GB/s", "cores": int(threads), "kind": "port",
-           "sample": f"OpenMP FP32 CSR SpMM (oracle), rows [{lb},{lb + done_rows}) of the partition "
+           "gflops": 2.0 * done_nnz * N / t_cpu / 1e9,
+           "sample": f"OpenMP FP32 CSR SpMM (oracle; gcc -O3, row loop built for AVX-512 / AVX2+FMA / x86-64 "
+                     f"and picked for the host CPU; schedule(dynamic,64); {threads} threads), "
+                     f"rows [{lb},{lb + done_rows}) of the partition "
                      f"({done_nnz} entries, {frac_rows:.0%} of its rows) on {threads} threads; "
                      + (f"median of {len(times)} passes after 2 warm-up passes ({t_cpu:.2f} s each)"
                         if done_rows == M else f"one pass over that sample ({t_cpu:.1f} s), "
@@ -525,8 +528,15 @@ def main():
                    "row_order": args.row_order, "tuning": tuning, "tuning_source": tuning_source},
         "roofline": {"bound": "hbm", "achieved": round(k_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(k_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_GBps": round(traffic / (spmm_ms * 1e-3) / 1e9, 1) if traffic else None,
-                     "traffic_frac": round(traffic / (spmm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
+                     # The counters measure bytes that miss the L2 (TCC): HBM
+                     # reads plus hits in the MALL (Infinity Cache), which
+                     # the counters do not separate, so the rate below is an
+                     # L2-miss rate, not an HBM rate.
+                     "traffic_kind": "L2 (TCC) miss bytes per call, MALL hits included: FETCH_SIZE x calibration "
+                                     "+ WRITE_SIZE, from rocprofv3 --pmc runs of this workload and schedule",
+                     "traffic_source": (f"profiles/{os.path.basename(tfile)} (tuning {prof['tuning']})"
+                                        if traffic else None),
+                     "l2_miss_GBps": round(traffic / (spmm_ms * 1e-3) / 1e9, 1) if traffic else None,
                      "kernel": "plan_rows_kernel (+ hub_reduce_kernel), one cola_spmm_partition call",
                      "kernel_ms": round(spmm_ms, 4), "kernel_bytes_alg": kbytes},
         "cpu_baseline": cpu,

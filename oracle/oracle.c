@@ -198,6 +198,22 @@ float oracle_fp8_row_scale(const float *x, int n) {
 
 /* OpenMP FP32 SpMM over global rows [lb, ub): C = A[lb:ub,:]·B (overwrite).
  * Returns the thread count used. */
+/* One output row of the CPU baseline: c = sum_k val[k] * B[col[k], :].
+ * Built for AVX-512, AVX2+FMA and plain x86-64, picked at load time for the
+ * host CPU (the library is built once and runs on whichever host the GPU
+ * sits in), so the dim-wide inner loop runs in the widest vectors the host
+ * has. */
+__attribute__((target_clones("avx512f", "avx2", "default")))
+static void baseline_row_f32(const int32_t *restrict col_idx, const float *restrict val, int64_t k0,
+                             int64_t k1, const float *restrict B, int64_t ldB, int N, float *restrict c) {
+  for (int j = 0; j < N; ++j) c[j] = 0.f;
+  for (int64_t k = k0; k < k1; ++k) {
+    const float a = val[k];
+    const float *restrict b = B + (int64_t)col_idx[k] * ldB;
+    for (int j = 0; j < N; ++j) c[j] += a * b[j];
+  }
+}
+
 int oracle_spmm_omp_f32(const int64_t *row_ptr, const int32_t *col_idx,
                         const float *val, int64_t lb, int64_t ub, const float *B,
                         int64_t ldB, int N, float *C) {
@@ -210,15 +226,8 @@ int oracle_spmm_omp_f32(const int64_t *row_ptr, const int32_t *col_idx,
   }
 #pragma omp parallel for schedule(dynamic, 64)
 #endif
-  for (int64_t i = lb; i < ub; ++i) {
-    float *c = C + (i - lb) * (int64_t)N;
-    for (int j = 0; j < N; ++j) c[j] = 0.f;
-    for (int64_t k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
-      const float a = val[k];
-      const float *b = B + (int64_t)col_idx[k] * ldB;
-      for (int j = 0; j < N; ++j) c[j] += a * b[j];
-    }
-  }
+  for (int64_t i = lb; i < ub; ++i)
+    baseline_row_f32(col_idx, val, row_ptr[i], row_ptr[i + 1], B, ldB, N, C + (i - lb) * (int64_t)N);
   return threads;
 }
 
