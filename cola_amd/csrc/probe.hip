@@ -1,0 +1,130 @@
+// Measurement kernel: the gather ceiling of a plan's entry stream.
+//
+// gather_ceiling_kernel does only what any kernel computing A_p·B over the
+// plan's entries must do, in the simplest form: one group of G lanes per work
+// item, each lane holding 16 bytes (4 FP32 columns) of the column tile; the
+// group reads its entries (column, value) straight from memory (all G lanes
+// read the same word, which the hardware broadcasts), loads 8 B rows ahead,
+// multiply-adds into registers and stores the item's row once.  No staging,
+// no cross-lane broadcast, no compensated sums.  Timing it beside the plan
+// kernel on the same items and tiles bounds what any schedule of this entry
+// stream can gain.  Modes:
+//   0: the entries' real rows (local slice or packed buffer);
+//   1: each row index masked into the first `table_rows` rows of its table,
+//      a table that stays in L2 (same entry stream, same instructions);
+//   2: entry k reads row k mod table_rows of the local slice: the same number
+//      of row loads, streamed in address order (no gather at all).
+// FP32 only; the column tile must be a multiple of 4 columns <= 256.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "device.hpp"
+#include "rows.hpp"
+
+using namespace cola_devhost;
+using cola::fail;
+
+namespace {
+
+template <int G>
+__global__ __launch_bounds__(256) void gather_ceiling_kernel(const int4 *__restrict__ items, int n_items,
+                                                             const int32_t *__restrict__ cols,
+                                                             const float *__restrict__ vals,
+                                                             const float *__restrict__ Bl, int64_t ldB,
+                                                             const float *__restrict__ Bp, int64_t ldP,
+                                                             float *__restrict__ C, int64_t ldC,
+                                                             float *__restrict__ partials, int64_t pstride, int mode,
+                                                             int64_t table_rows) {
+  constexpr int U = 8;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int gl = (int)(tid % G);
+  const int64_t n_groups = (int64_t)gridDim.x * blockDim.x / G;
+  const int lc = gl * 4;
+  for (int64_t it = tid / G; it < n_items; it += n_groups) {
+    const int4 item = items[it];
+    const int b = item.y, e = item.z;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int k = b; k < e; k += U) {
+      float4 x[U];
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int kk = min(k + u, e - 1);
+        const int c = cols[kk];
+        v[u] = k + u < e ? vals[kk] : 0.f;
+        const float *rp;
+        if (mode == 2) {
+          rp = Bl + (int64_t)(kk % table_rows) * ldB;
+        } else if (c >= 0) {
+          rp = Bl + (int64_t)(mode == 1 ? c & (table_rows - 1) : c) * ldB;
+        } else {
+          rp = Bp + (int64_t)(mode == 1 ? (~c) & (table_rows - 1) : ~c) * ldP;
+        }
+        x[u] = *reinterpret_cast<const float4 *>(rp + lc);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        acc[0] = fmaf(v[u], x[u].x, acc[0]);
+        acc[1] = fmaf(v[u], x[u].y, acc[1]);
+        acc[2] = fmaf(v[u], x[u].z, acc[2]);
+        acc[3] = fmaf(v[u], x[u].w, acc[3]);
+      }
+    }
+    float *dst = item.x >= 0 ? C + (int64_t)item.x * ldC : partials + (int64_t)(-item.x - 1) * pstride;
+    *reinterpret_cast<float4 *>(dst + lc) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+  }
+}
+
+}  // namespace
+
+extern "C" cola_status cola_probe_gather(cola_plan p, const float *B_local, int64_t ld_B, const float *packed,
+                                         int64_t ld_packed, float *C, int64_t ld_C, int dim, int col_tile, int mode,
+                                         int64_t table_rows, int blocks_per_cu, void *stream) {
+  if (!p || !C || dim < 4 || dim % 4 || ld_C < dim || ld_B < dim || (p->U_r > 0 && ld_packed < dim))
+    return fail(COLA_ERR_INVALID, "probe_gather: bad arguments");
+  if ((p->U_l > 0 && !B_local) || (p->U_r > 0 && !packed)) return fail(COLA_ERR_INVALID, "probe_gather: NULL B");
+  if (!aligned(C, 16) || ld_C % 4 || (B_local && (!aligned(B_local, 16) || ld_B % 4)) ||
+      (packed && (!aligned(packed, 16) || ld_packed % 4)))
+    return fail(COLA_ERR_INVALID, "probe_gather: matrices must allow 16-byte row vectors");
+  if (mode < 0 || mode > 2 || (mode > 0 && table_rows < 1) || (mode == 1 && (table_rows & (table_rows - 1))))
+    return fail(COLA_ERR_INVALID, "probe_gather: mode 1 needs a power-of-two table_rows, mode 2 table_rows >= 1");
+  if (mode == 1 && (table_rows > p->nodesPerPE || (p->U_r > 0 && table_rows > p->U_r)))
+    return fail(COLA_ERR_INVALID, "probe_gather: table_rows exceeds a table");
+  if (mode == 2 && table_rows > p->nodesPerPE) return fail(COLA_ERR_INVALID, "probe_gather: table_rows > nodesPerPE");
+  if (dim > p->max_dim && p->num_chunks > 0) return fail(COLA_ERR_INVALID, "probe_gather: dim above max_dim");
+  const int tile = col_tile > 0 && col_tile < dim ? col_tile : dim;
+  if (tile % 4 || tile > 256) return fail(COLA_ERR_INVALID, "probe_gather: tile must be a multiple of 4, <= 256");
+  hipStream_t s = (hipStream_t)stream;
+  const int bpc = blocks_per_cu > 0 ? blocks_per_cu : 8;
+  for (int c0 = 0; c0 < dim; c0 += tile) {
+    const int w = std::min(tile, dim - c0);
+    if (w % 4) return fail(COLA_ERR_INVALID, "probe_gather: dim must be a multiple of the tile's vector width");
+    const int G = w / 4;
+    for (const PlanBin *bin : {&p->longb, &p->shortb}) {
+      if (bin->n == 0) continue;
+      const int64_t blocks = std::min<int64_t>((bin->n * G + 255) / 256, (int64_t)device_cu_count() * bpc);
+#define COLA_PROBE_CASE(GG)                                                                                 \
+  case GG:                                                                                                  \
+    hipLaunchKernelGGL((gather_ceiling_kernel<GG>), dim3((unsigned)blocks), dim3(256), 0, s, bin->items,     \
+                       (int)bin->n, p->cols, p->vals, B_local ? B_local + c0 : nullptr, ld_B,               \
+                       packed ? packed + c0 : nullptr, ld_packed, C + c0, ld_C,                             \
+                       p->partials ? p->partials + c0 : nullptr, p->pstride, mode, table_rows);             \
+    break;
+      switch (G) {
+        COLA_PROBE_CASE(1)
+        COLA_PROBE_CASE(2)
+        COLA_PROBE_CASE(4)
+        COLA_PROBE_CASE(8)
+        COLA_PROBE_CASE(16)
+        COLA_PROBE_CASE(32)
+        COLA_PROBE_CASE(64)
+        default:
+          return fail(COLA_ERR_UNSUPPORTED, "probe_gather: tile width must be 4 x a power of two columns");
+      }
+#undef COLA_PROBE_CASE
+    }
+  }
+  return hip_check(hipGetLastError(), "probe_gather launch");
+}

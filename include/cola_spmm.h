@@ -576,6 +576,26 @@ cola_status cola_error_norms(const float *C, int64_t ld_C, const double *R,
                              int64_t ld_R, int64_t rows, int dim, double *out,
                              void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Measurement.                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* No counterpart in the reference.  The gather ceiling of a plan's entry
+ * stream: a minimal FP32 kernel that reads each item's entries, loads the B
+ * rows they name (16 bytes per lane, 8 rows in flight per lane group),
+ * multiply-adds into registers and stores each row once, over the plan's
+ * long-row then short-row items, one launch per column tile of col_tile
+ * columns (0 = all; a multiple of 4, at most 256).  mode 0: the real rows;
+ * 1: row indices masked into the first table_rows (a power of two) rows of
+ * each table, so the tables stay in L2; 2: entry k reads local row
+ * k mod table_rows (the same row loads in address order, no gather).
+ * Overwrites C (and the hub chunk partials, not reduced).  For timing only. */
+cola_status cola_probe_gather(cola_plan plan, const float *B_local, int64_t ld_B,
+                              const float *packed, int64_t ld_packed, float *C,
+                              int64_t ld_C, int dim, int col_tile, int mode,
+                              int64_t table_rows, int blocks_per_cu,
+                              void *stream);
+
 #ifdef __cplusplus
 }
 #endif
