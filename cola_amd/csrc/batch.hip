@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <new>
@@ -226,6 +227,7 @@ struct cola_batch_s {
   BatchSeg *d_segs = nullptr;
   int n_segs = 0, total_tasks = 0, seg_cap = 0;
   XcdBounds xb{};            // task range of each XCD
+  std::vector<SegCost> costs;     // cost model of the current segments
   HubSeg *d_hubs = nullptr;
   int n_hubs = 0, hub_cap = 0;
   int64_t hub_threads = 0;
@@ -375,6 +377,7 @@ static cola_status build_tables(cola_batch_s *b) {
     if (st) return st;
   }
   b->xb = xcd_bounds(costs, task);
+  b->costs = costs;
   if ((int)segs.size() > b->seg_cap) {
     (void)hipFree(b->d_segs);
     b->d_segs = nullptr;
@@ -627,6 +630,102 @@ cola_status cola_batch_autotune(cola_batch b, int iters, void *stream, float *be
       }
     }
     b->blocks_per_cu = best_bpc;
+  }
+  // Re-cut the XCD ranges by measured time: each XCD's range is timed alone
+  // (the other XCDs' ranges empty), its time per unit of modelled cost is
+  // spread over its tasks, and the ranges are cut again at equal shares of
+  // that measured time; two rounds, kept only if the whole batch runs faster.
+  if (!st && b->n_segs > 0 && b->total_tasks > 0) {
+    auto time_rows = [&](const XcdBounds &xb, float *ms) -> cola_status {
+      auto go = [&]() {
+        int64_t blocks = std::min<int64_t>(((int64_t)b->total_tasks + 3) / 4,
+                                           (int64_t)device_cu_count() * b->blocks_per_cu);
+        blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
+        const int acc = (b->flags & COLA_OVERWRITE) ? 0 : 1;
+        const bool kh = (b->flags & COLA_KAHAN) != 0;
+        switch (b->dtype) {
+          case COLA_F32: launch_batch<COLA_F32, 4>(b->d_segs, b->n_segs, xb, acc, kh, (int)blocks, s); break;
+          case COLA_BF16: launch_batch<COLA_BF16, 8>(b->d_segs, b->n_segs, xb, acc, kh, (int)blocks, s); break;
+          case COLA_F16: launch_batch<COLA_F16, 8>(b->d_segs, b->n_segs, xb, acc, kh, (int)blocks, s); break;
+          case COLA_FP8_E4M3: launch_batch<COLA_FP8_E4M3, 16>(b->d_segs, b->n_segs, xb, acc, kh, (int)blocks, s); break;
+        }
+      };
+      go();
+      cola_status r;
+      if ((r = hip_check(hipEventRecord(e0, s), "hipEventRecord"))) return r;
+      for (int i = 0; i < iters; ++i) go();
+      if ((r = hip_check(hipEventRecord(e1, s), "hipEventRecord"))) return r;
+      if ((r = hip_check(hipEventSynchronize(e1), "hipEventSynchronize"))) return r;
+      float t = 0.f;
+      if ((r = hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime"))) return r;
+      *ms = t / iters;
+      return hip_check(hipGetLastError(), "batch XCD timing launch");
+    };
+    // Modelled cost of the first k tasks of the whole list.
+    std::vector<double> seg_before(b->costs.size() + 1, 0.0);
+    for (size_t i = 0; i < b->costs.size(); ++i)
+      seg_before[i + 1] = seg_before[i] + seg_cost_before(b->costs[i], b->costs[i].task_end - b->costs[i].task_begin);
+    auto cost_to = [&](int64_t t) {
+      size_t i = 0;
+      while (i + 1 < b->costs.size() && b->costs[i].task_end <= t) ++i;
+      const SegCost &c = b->costs[i];
+      return seg_before[i] + seg_cost_before(c, std::max<int64_t>(0, std::min<int64_t>(t, c.task_end) - c.task_begin));
+    };
+    const XcdBounds model_xb = b->xb;
+    XcdBounds cur = b->xb, best_xb = b->xb;
+    for (int round = 0; round < 2 && !st; ++round) {
+      double t_x[8];
+      for (int x = 0; x < 8 && !st; ++x) {
+        XcdBounds one{};
+        for (int y = 0; y <= 8; ++y) one.v[y] = y <= x ? cur.v[x] : cur.v[x + 1];
+        float ms = 0.f;
+        st = time_rows(one, &ms);
+        t_x[x] = ms;
+      }
+      if (st) break;
+      if (verbose) {
+        fprintf(stderr, "[batch autotune] XCD ranges alone (round %d):", round);
+        for (int x = 0; x < 8; ++x) fprintf(stderr, " %.3f", t_x[x]);
+        fprintf(stderr, " ms\n");
+      }
+      // measured time of the first t tasks: piecewise by range
+      double c_x[9], m_x[9];
+      for (int x = 0; x <= 8; ++x) c_x[x] = cost_to(cur.v[x]);
+      m_x[0] = 0.0;
+      for (int x = 0; x < 8; ++x) m_x[x + 1] = m_x[x] + t_x[x];
+      auto time_to = [&](int64_t t) {
+        int x = 0;
+        while (x < 7 && cur.v[x + 1] <= t) ++x;
+        const double span = c_x[x + 1] - c_x[x];
+        const double f = span > 0 ? (cost_to(t) - c_x[x]) / span : 0.0;
+        return m_x[x] + f * t_x[x];
+      };
+      XcdBounds nxt{};
+      nxt.v[0] = 0;
+      nxt.v[8] = b->total_tasks;
+      for (int x = 1; x < 8; ++x) {
+        const double target = m_x[8] * x / 8.0;
+        int64_t lo = nxt.v[x - 1], hi = b->total_tasks;
+        while (lo < hi) {
+          const int64_t mid = (lo + hi) / 2;
+          if (time_to(mid) >= target) hi = mid;
+          else lo = mid + 1;
+        }
+        nxt.v[x] = (int32_t)lo;
+      }
+      b->xb = nxt;
+      float ms = 0.f;
+      if ((st = measure(&ms))) break;
+      if (verbose) fprintf(stderr, "[batch autotune] XCD ranges re-cut by measured time: %.4f ms (best %.4f)\n", ms, bms);
+      if (ms < bms * 0.995f) {
+        bms = ms;
+        best_xb = nxt;
+      }
+      cur = nxt;
+    }
+    b->xb = best_xb;
+    if (verbose && std::memcmp(&best_xb, &model_xb, sizeof(XcdBounds)) != 0)
+      fprintf(stderr, "[batch autotune] keeping the XCD ranges cut by measured time\n");
   }
   if (!st) st = run_batch(b, s);
   if (!st) st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
