@@ -59,7 +59,7 @@ class PlanOpts(ctypes.Structure):
 class PlanTuning(ctypes.Structure):
     _fields_ = [("col_tile", c_i32), ("rpw_short", c_i32), ("rpw_long", c_i32), ("blocks_per_cu", c_i32),
                 ("xcd_remap", c_i32), ("split_remote", c_i32),
-                ("unroll", c_i32), ("panels", c_i32), ("blocks_per_cu_short", c_i32)]
+                ("unroll", c_i32), ("panels", c_i32), ("blocks_per_cu_short", c_i32), ("tiny", c_i32)]
 
 
 class BatchProblem(ctypes.Structure):
@@ -80,7 +80,7 @@ class PlanInfo(ctypes.Structure):
     _fields_ = [("num_rows", c_i64), ("num_items", c_i64), ("num_hub_rows", c_i64),
                 ("num_hub_chunks", c_i64), ("nnz", c_i64), ("U_r", c_i64), ("U_l", c_i64),
                 ("device_bytes", c_i64), ("num_short_items", c_i64), ("num_long_items", c_i64),
-                ("hot_nnz", c_i64), ("num_cold_items", c_i64)]
+                ("hot_nnz", c_i64), ("num_cold_items", c_i64), ("num_tiny_bundles", c_i64)]
 
 
 # name -> argtypes (all return cola_status unless listed in _RESTYPES)

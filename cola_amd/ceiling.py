@@ -24,6 +24,7 @@ def main():
     ap.add_argument("--only", default=None, choices=["plan", "real", "l2_resident", "streamed"],
                     help="run only this kernel (for counter runs)")
     ap.add_argument("--bpc", type=int, default=16, help="probe grid in blocks per CU with --only")
+    ap.add_argument("--plan-opts", default=None, help="JSON of plan options (e.g. hot_cols, col_panels)")
     args = ap.parse_args()
 
     import torch
@@ -37,7 +38,7 @@ def main():
     g = cola.Graph.rmat(args.log_n, args.degree, seed=2, reorder="community")
     part = g.partition(args.nparts, 0)
     P = part.info()["nodesPerPE"]
-    plan = cola.Plan(part, row_order=1)
+    plan = cola.Plan(part, **dict(dict(row_order=1), **json.loads(args.plan_opts or "{}")))
     N = args.dim
     gen = torch.Generator(device=dev).manual_seed(1234)
     B = torch.rand((P * args.nparts, N), generator=gen, device=dev).mul_(2).sub_(1)

@@ -90,6 +90,16 @@ struct HostPlan {
   std::vector<int32_t> panel_xcd;       // 2K x 9 item ranges (relative to the panel)
   std::vector<int32_t> panel_hub_ptr;   // chunk range per hub (num_hubs+1)
   int32_t panel_chunks = 0;
+  // Tiny-row bundles (one-pass plans): rows of L <= 4 entries without the
+  // compensated-sum flag, grouped by L into bundles of floor(8/L) rows whose
+  // entries are copied back to back after the partition's entries, so one
+  // lane group loads a whole bundle in one batch of row loads.  A bundle item
+  // is (offset of its rows in tiny_rows, begin, end, L).  items_short_nt holds
+  // the short-row items that are not in a bundle (the short bin of a
+  // schedule that runs the bundles).
+  std::vector<WorkItem> items_tiny, items_short_nt;
+  std::vector<int32_t> xcd_tiny, xcd_short_nt;
+  std::vector<int32_t> tiny_rows;
 };
 
 cola_status build_plan(const Partition &p, const cola_plan_opts &opts,

@@ -63,6 +63,9 @@ struct cola_plan_s {
   PlanBin shortb_r, longb_r;  // items with remote entries (second phase)
   PlanBin emptyb;             // rows without entries as empty items (batched overwrite)
   PlanBin cshortb, clongb;    // second (cold) pass of a hot/cold plan, else empty
+  PlanBin tinyb;              // tiny-row bundles (rows of <= 4 entries), else empty
+  PlanBin shortb_nt;          // short rows outside the bundles
+  int32_t *tiny_rows = nullptr;  // output rows of the bundles
   int64_t hot_nnz = 0;
   int32_t *cols = nullptr, *empty_rows = nullptr, *hub_rows = nullptr, *hub_ptr = nullptr,
           *remote_cols = nullptr;
@@ -75,5 +78,5 @@ struct cola_plan_s {
   int32_t *panel_xcd = nullptr, *panel_hub_ptr = nullptr;
   std::vector<int64_t> panel_begin;
   int64_t panel_chunks = 0;
-  cola_plan_tuning tune{0, 0, 1, 8, 1, 0, 8, 0, 0};  // set by size at creation
+  cola_plan_tuning tune{0, 0, 1, 8, 1, 0, 8, 0, 0, 1};  // set by size at creation
 };

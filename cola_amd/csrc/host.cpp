@@ -706,6 +706,49 @@ cola_status build_plan(const Partition &p, const cola_plan_opts &o,
     }
     if (out->panel_items.size() > (size_t)INT32_MAX) return fail(COLA_ERR_INVALID, "plan: too many panel items");
   }
+
+  // Tiny-row bundles: short rows of L <= 4 entries (not flagged for
+  // compensated sums) in item order, grouped by L, floor(8/L) rows a bundle.
+  // A row of one entry would otherwise occupy a lane group for a whole batch
+  // of 8 row loads, 7 of them padding.  The entries of a bundle are copied
+  // back to back after the partition's own; the copies keep each row's entry
+  // order, so a row's sum is formed in the same order as in the short bin.
+  out->items_tiny.clear();
+  out->items_short_nt.clear();
+  out->tiny_rows.clear();
+  if (H == 0) {
+    constexpr int kTinyMax = 4, kBatch = 8;
+    std::vector<WorkItem> byL[kTinyMax + 1];
+    for (const WorkItem &w : out->items_short) {
+      const int64_t d = (int64_t)w.end - w.begin;
+      if (d >= 1 && d <= kTinyMax && (w.flags & 1) == 0) byL[d].push_back(w);
+      else out->items_short_nt.push_back(w);
+    }
+    for (int L = kTinyMax; L >= 1; --L) {
+      const int R = kBatch / L;
+      for (size_t i = 0; i < byL[L].size(); i += R) {
+        const size_t n = std::min<size_t>(R, byL[L].size() - i);
+        const int64_t b = (int64_t)out->cols.size();
+        if (b + (int64_t)n * L > INT32_MAX || out->tiny_rows.size() + n > (size_t)INT32_MAX)
+          return fail(COLA_ERR_INVALID, "plan: tiny-row bundles overflow int32 offsets");
+        const int32_t roff = (int32_t)out->tiny_rows.size();
+        for (size_t k = 0; k < n; ++k) {
+          const WorkItem &w = byL[L][i + k];
+          out->tiny_rows.push_back(w.out);
+          for (int32_t q = w.begin; q < w.end; ++q) {
+            const int32_t c = out->cols[q];
+            const float v = out->vals[q];
+            out->cols.push_back(c);
+            out->vals.push_back(v);
+          }
+        }
+        out->items_tiny.push_back({roff, (int32_t)b, (int32_t)(b + (int64_t)n * L), L});
+      }
+    }
+    if (out->items_tiny.empty()) out->items_short_nt.clear();
+  }
+  split(out->items_tiny, &out->xcd_tiny);
+  split(out->items_short_nt, &out->xcd_short_nt);
   return COLA_OK;
 }
 

@@ -528,6 +528,154 @@ __global__ __launch_bounds__(256) void plan_rows_kernel(const RowArgs a) {
     row_step<DT, EPL, G, NV, U, KAHAN, kPlanPipe<DT>>(a, it0, ie, rpw);
 }
 
+// One bundle of tiny rows per lane group: R = 8 / L rows of L entries each
+// (the last bundle of a length may hold fewer), entries back to back at
+// [b, e).  Group lane u < R*L stages entry b + u (coalesced) and turns it
+// into its B row's address and value, the group broadcasts the R*L pairs
+// with ds_bpermute and issues all R*L row loads before any FMA, and row j
+// sums slots j*L .. j*L+L-1 in entry order (the short bin's order for a row
+// that one lane group sums alone).  One 16-byte vector per lane (NV = 1).
+template <int DT, int EPL, int G, int L>
+__device__ __forceinline__ void tiny_bundle(const RowArgs &a, const int32_t *__restrict__ rowtab, const int4 item,
+                                            bool has, int gl, int tbase, uint64_t lbase, int lcol) {
+  using V = Vec<DT, EPL>;
+  using T = typename V::T;
+  using Raw = typename V::Raw;
+  constexpr int R = 8 / L, N = R * L;
+  static_assert(N <= G, "a bundle's entries are staged one per group lane");
+  const T *__restrict__ Bl = static_cast<const T *>(a.Bl);
+  const T *__restrict__ Bp = static_cast<const T *>(a.Bp);
+  const int b = item.y, n = has ? item.z - item.y : 0;  // n = rows * L <= N
+  // Stage: lane gl < n takes entry b + gl; the others re-read entry b with
+  // value 0 so that every broadcast slot holds a valid row address.
+  uint32_t mlo = 0, mhi = 0;
+  float mv = 0.f;
+  if (has) {
+    const int k = b + (gl < n ? gl : 0);
+    const int c = gld(a.cols + k);
+    const float x = gld(a.vals + k);
+    const bool loc = c >= 0;
+    const T *rp = loc ? Bl + (int64_t)c * a.ldB : Bp + (int64_t)(~c) * a.ldP;
+    const uint64_t p = reinterpret_cast<uint64_t>(rp);
+    if (a.abase) {
+      mlo = (uint32_t)((p - a.abase) >> 4);
+    } else {
+      mlo = (uint32_t)p;
+      mhi = (uint32_t)(p >> 32);
+    }
+    mv = gl < n ? x : 0.f;
+    if constexpr (DT == COLA_FP8_E4M3) mv *= gld(loc ? a.sl + c : a.sp + ~c);
+  }
+  if (n == 0) return;  // past the XCD range's last bundle (uniform over the group)
+  float av[N];
+  Raw raw[N];
+  if (a.abase) {
+    uint32_t ou[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      const int src = (tbase + u) << 2;
+      ou[u] = __builtin_amdgcn_ds_bpermute(src, (int)mlo);
+      av[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv)));
+    }
+#pragma unroll
+    for (int u = 0; u < N; ++u) raw[u] = V::gload(lbase + ((uint64_t)ou[u] << 4));
+  } else {
+    uint64_t pu[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      const int src = (tbase + u) << 2;
+      const uint32_t lo = __builtin_amdgcn_ds_bpermute(src, (int)mlo);
+      const uint32_t hi = __builtin_amdgcn_ds_bpermute(src, (int)mhi);
+      pu[u] = ((uint64_t)hi << 32) | lo;
+      av[u] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(mv)));
+    }
+#pragma unroll
+    for (int u = 0; u < N; ++u) raw[u] = V::gload(pu[u] + (uint64_t)lcol * sizeof(T));
+  }
+  // Output rows of the bundle: lane j < R reads row j, broadcast below.
+  const int rows = n / L;
+  const int rid = gl < rows ? gld(rowtab + item.x + gl) : 0;
+  __builtin_amdgcn_sched_barrier(0);
+  // Slots past the bundle's entries hold a real row with value 0: zero them
+  // so an Inf or NaN there cannot reach a sum.
+  if (n < N) {
+#pragma unroll
+    for (int u = 0; u < N; ++u)
+      if (u >= n) raw[u] = V::zero();
+  }
+  float acc[R][EPL];
+#pragma unroll
+  for (int j = 0; j < R; ++j)
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) acc[j][q] = 0.f;
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    float x[EPL];
+    V::widen(raw[u], x);
+#pragma unroll
+    for (int q = 0; q < EPL; ++q) acc[u / L][q] = fmaf(av[u], x[q], acc[u / L][q]);
+  }
+  const bool lane_ok = gl * EPL < a.dim;  // lanes past the tile's columns store nothing
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int row = __shfl(rid, tbase + j);
+    if (j < rows && lane_ok) {
+      float *dst = a.C + (int64_t)row * a.ldC + lcol;
+#pragma unroll
+      for (int q = 0; q < EPL; q += 4) {
+        float4 o = make_float4(acc[j][q], acc[j][q + 1], acc[j][q + 2], acc[j][q + 3]);
+        if (a.accumulate) {
+          const float4 old = gld(reinterpret_cast<const float4 *>(dst + q));
+          o.x += old.x; o.y += old.y; o.z += old.z; o.w += old.w;
+        }
+        gst(reinterpret_cast<float4 *>(dst + q), o);
+      }
+    }
+  }
+}
+
+// Tiny-row bundles (plan items (row offset, begin, end, L)): one bundle per
+// lane group per step, items walked like plan_rows_kernel's (XCD ranges).
+template <int DT, int EPL, int G>
+__global__ __launch_bounds__(256) void tiny_rows_kernel(const RowArgs a, const int32_t *__restrict__ rowtab) {
+  using T = typename Vec<DT, EPL>::T;
+  constexpr int S = 64 / G;
+  const int wpb = blockDim.x >> 6;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int blk, nblk, ib, ie;
+  if (a.xcd_remap) {
+    const int xcd = blockIdx.x & 7;
+    blk = blockIdx.x >> 3;
+    nblk = (gridDim.x - xcd + 7) >> 3;
+    ib = a.xcd_begin[xcd];
+    ie = a.xcd_begin[xcd + 1];
+  } else {
+    blk = blockIdx.x;
+    nblk = gridDim.x;
+    ib = a.xcd_begin[0];
+    ie = a.xcd_begin[8];
+  }
+  const int lane = threadIdx.x & 63;
+  const int sub = lane / G, gl = lane % G, tbase = sub * G;
+  const int col = gl * EPL;
+  const int lcol = col < a.dim ? col : 0;
+  const uint64_t lbase = a.abase + (uint64_t)lcol * sizeof(T);
+  const int wave = blk * wpb + wid, nwave = nblk * wpb;
+  for (int it0 = ib + wave * S; it0 < ie; it0 += nwave * S) {
+    const int it = it0 + sub;
+    const bool has = it < ie;
+    const int4 item = has ? gld(a.items + it) : make_int4(0, 0, 0, 4);
+    // The length is uniform over a lane group; groups of one wave differ only
+    // where the bundles of one length end and the next begin.
+    switch (item.w) {
+      case 1: tiny_bundle<DT, EPL, G, 1>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
+      case 2: tiny_bundle<DT, EPL, G, 2>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
+      case 3: tiny_bundle<DT, EPL, G, 3>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
+      default: tiny_bundle<DT, EPL, G, 4>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
+    }
+  }
+}
+
 // Lanes per row group and vectors per lane for a dim.  vec_ok: dim, leading
 // dimensions and base pointers allow 16-byte vectors.
 struct Shape {
@@ -578,6 +726,30 @@ bool launch_rows_f16(const Shape &sh, const RowArgs &a, int blocks, int threads,
                       bool unroll16);
 bool launch_rows_fp8(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s,
                       bool unroll16);
+
+// Launches the tiny-row bundle kernel; returns false when the shape has no
+// bundle form (one 16-byte vector per lane and at least 8 lanes per group).
+bool launch_tiny_f32(const Shape &sh, const RowArgs &a, const int32_t *rowtab, int blocks, hipStream_t s);
+bool launch_tiny_bf16(const Shape &sh, const RowArgs &a, const int32_t *rowtab, int blocks, hipStream_t s);
+bool launch_tiny_f16(const Shape &sh, const RowArgs &a, const int32_t *rowtab, int blocks, hipStream_t s);
+bool launch_tiny_fp8(const Shape &sh, const RowArgs &a, const int32_t *rowtab, int blocks, hipStream_t s);
+
+inline bool tiny_shape_ok(int dt, const Shape &sh) {
+  const int ev = dt == COLA_F32 ? 4 : (dt == COLA_FP8_E4M3 ? 16 : 8);
+  return sh.epl == ev && sh.nv == 1 && sh.g >= 8;
+}
+
+template <int DT, int EV>
+bool launch_tiny_impl(const Shape &sh, const RowArgs &a, const int32_t *rowtab, int blocks, hipStream_t s) {
+  if (sh.epl != EV || sh.nv != 1) return false;
+  switch (sh.g) {
+    case 8: hipLaunchKernelGGL((tiny_rows_kernel<DT, EV, 8>), dim3(blocks), dim3(256), 0, s, a, rowtab); return true;
+    case 16: hipLaunchKernelGGL((tiny_rows_kernel<DT, EV, 16>), dim3(blocks), dim3(256), 0, s, a, rowtab); return true;
+    case 32: hipLaunchKernelGGL((tiny_rows_kernel<DT, EV, 32>), dim3(blocks), dim3(256), 0, s, a, rowtab); return true;
+    case 64: hipLaunchKernelGGL((tiny_rows_kernel<DT, EV, 64>), dim3(blocks), dim3(256), 0, s, a, rowtab); return true;
+  }
+  return false;
+}
 
 template <int DT, int EV>
 bool launch_rows_impl(const Shape &sh, const RowArgs &a, int blocks, int threads, hipStream_t s,

@@ -8,4 +8,7 @@ bool launch_rows_bf16(const Shape &sh, const RowArgs &a, int blocks, int threads
                       bool unroll16) {
   return launch_rows_impl<COLA_BF16, 8>(sh, a, blocks, threads, s, unroll16);
 }
+bool launch_tiny_bf16(const Shape &sh, const RowArgs &a, const int32_t *rowtab, int blocks, hipStream_t s) {
+  return launch_tiny_impl<COLA_BF16, 8>(sh, a, rowtab, blocks, s);
+}
 }  // namespace cola_dev

@@ -376,7 +376,8 @@ static void free_zip(cola_zip_s *z) {
 
 static void free_plan(cola_plan_s *p) {
   if (!p) return;
-  for (PlanBin *b : {&p->shortb, &p->longb, &p->shortb_r, &p->longb_r, &p->emptyb, &p->cshortb, &p->clongb}) {
+  for (PlanBin *b : {&p->shortb, &p->longb, &p->shortb_r, &p->longb_r, &p->emptyb, &p->cshortb, &p->clongb,
+                     &p->tinyb, &p->shortb_nt}) {
     (void)hipFree(b->items);
     (void)hipFree(b->xcd);
   }
@@ -390,6 +391,7 @@ static void free_plan(cola_plan_s *p) {
   (void)hipFree(p->panel_items);
   (void)hipFree(p->panel_xcd);
   (void)hipFree(p->panel_hub_ptr);
+  (void)hipFree(p->tiny_rows);
   delete p;
 }
 
@@ -415,6 +417,17 @@ bool launch_rows(int dt, const Shape &sh, const cola_dev::RowArgs &a, int blocks
     case COLA_BF16: return cola_dev::launch_rows_bf16(sh, a, blocks, 256, s, u16);
     case COLA_F16: return cola_dev::launch_rows_f16(sh, a, blocks, 256, s, u16);
     case COLA_FP8_E4M3: return cola_dev::launch_rows_fp8(sh, a, blocks, 256, s, u16);
+  }
+  return false;
+}
+
+bool launch_tiny(int dt, const Shape &sh, const cola_dev::RowArgs &a, const int32_t *rowtab, int blocks,
+                 hipStream_t s) {
+  switch (dt) {
+    case COLA_F32: return cola_dev::launch_tiny_f32(sh, a, rowtab, blocks, s);
+    case COLA_BF16: return cola_dev::launch_tiny_bf16(sh, a, rowtab, blocks, s);
+    case COLA_F16: return cola_dev::launch_tiny_f16(sh, a, rowtab, blocks, s);
+    case COLA_FP8_E4M3: return cola_dev::launch_tiny_fp8(sh, a, rowtab, blocks, s);
   }
   return false;
 }
@@ -563,10 +576,14 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
     // inside this column tile so the hot rows stay in L2.
     const bool cold = p->cshortb.n + p->clongb.n > 0;
     const bool split = t.split_remote && p->U_r > 0;
+    // Tiny-row bundles replace their rows' short-bin items in a one-pass
+    // schedule whose lane groups hold one 16-byte vector and span >= 8 lanes.
+    const bool tiny = t.tiny && !split && p->tinyb.n > 0 && cola_dev::tiny_shape_ok(c.dtype, sh);
     for (int phase : {split ? 1 : 0, split ? 2 : -1, cold ? 3 : -1}) {
       if (phase < 0) continue;
       const PlanBin *bins[2] = {phase == 2 ? &p->longb_r : (phase == 3 ? &p->clongb : &p->longb),
-                                phase == 2 ? &p->shortb_r : (phase == 3 ? &p->cshortb : &p->shortb)};
+                                phase == 2 ? &p->shortb_r
+                                           : (phase == 3 ? &p->cshortb : (tiny ? &p->shortb_nt : &p->shortb))};
       for (int bi = 0; bi < 2; ++bi) {
         const PlanBin *bin = bins[bi];
         if (bin->n == 0) continue;
@@ -599,6 +616,36 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
         blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
         if (!launch_rows(c.dtype, sh, a, (int)blocks, s, t.unroll == 16))
           return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: no kernel for this dim");
+      }
+      if (tiny && phase == 0) {
+        cola_dev::RowArgs a;
+        a.items = p->tinyb.items;
+        a.xcd_begin = p->tinyb.xcd;
+        a.cols = p->cols;
+        a.vals = p->vals;
+        a.Bl = c.Bl ? (const char *)c.Bl + (size_t)c0 * eb : nullptr;
+        a.ldB = c.ldB;
+        a.sl = c.sl;
+        a.Bp = c.Bp ? (const char *)c.Bp + (size_t)c0 * eb : nullptr;
+        a.ldP = c.ldP;
+        a.sp = c.sp;
+        a.C = c.C + c0;
+        a.ldC = c.ldC;
+        a.partials = nullptr;
+        a.pstride = 0;
+        a.dim = w;
+        a.accumulate = accumulate;
+        a.kahan = 0;
+        a.rpw = S;
+        a.xcd_remap = t.xcd_remap ? 1 : 0;
+        a.phase = 0;
+        a.abase = table_base(c0, w);
+        const int64_t waves = (p->tinyb.n + S - 1) / S;
+        const int bin_bpc = t.blocks_per_cu_short > 0 ? t.blocks_per_cu_short : bpc;
+        int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bin_bpc);
+        blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
+        if (!launch_tiny(c.dtype, sh, a, p->tiny_rows, (int)blocks, s))
+          return fail(COLA_ERR_UNSUPPORTED, "spmm_partition: no tiny-row kernel for this dim");
       }
     }
   }
@@ -852,8 +899,8 @@ static cola_status plan_from_partition(const cola::Partition &part, const cola_p
   // than the L2s hold and want 256-byte column tiles, 4 rows per wave at 2
   // blocks per CU and each XCD on its own item range; small ones want every
   // column in one pass and many waves.
-  p->tune = hp.nnz >= (int64_t(1) << 22) ? cola_plan_tuning{-2, 0, 4, 2, 1, 0, 8, 0, 4}
-                                         : cola_plan_tuning{0, 0, 1, 8, 1, 0, 8, 0, 0};
+  p->tune = hp.nnz >= (int64_t(1) << 22) ? cola_plan_tuning{-2, 0, 4, 2, 1, 0, 8, 0, 4, 1}
+                                         : cola_plan_tuning{0, 0, 1, 8, 1, 0, 8, 0, 0, 1};
   p->num_empty = (int64_t)hp.empty_rows.size();
   p->num_hubs = (int64_t)hp.hub_rows.size();
   p->num_chunks = hp.hub_chunk_ptr.back();
@@ -883,6 +930,18 @@ static cola_status plan_from_partition(const cola::Partition &part, const cola_p
       (st = upload(hp.hub_chunk_ptr, &p->hub_ptr)) || (st = upload(hp.remote_cols, &p->remote_cols))) {
     free_plan(p);
     return st;
+  }
+  {
+    std::vector<int4> it = to_int4(hp.items_tiny), isn = to_int4(hp.items_short_nt);
+    p->tinyb.n = (int64_t)it.size();
+    p->shortb_nt.n = (int64_t)isn.size();
+    if ((st = upload(it, &p->tinyb.items)) || (st = upload(hp.xcd_tiny, &p->tinyb.xcd)) ||
+        (st = upload(isn, &p->shortb_nt.items)) || (st = upload(hp.xcd_short_nt, &p->shortb_nt.xcd)) ||
+        (st = upload(hp.tiny_rows, &p->tiny_rows))) {
+      free_plan(p);
+      return st;
+    }
+    p->device_bytes += (int64_t)((it.size() + isn.size()) * sizeof(int4) + 18 * 4 + hp.tiny_rows.size() * 4);
   }
   auto cost_prefix = [](const std::vector<int4> &v) {
     std::vector<int64_t> c(v.size() + 1, 0);
@@ -1014,6 +1073,7 @@ cola_status cola_plan_get_info(cola_plan p, cola_plan_info *info) {
   info->num_long_items = p->longb.n;
   info->hot_nnz = p->hot_nnz;
   info->num_cold_items = p->cshortb.n + p->clongb.n;
+  info->num_tiny_bundles = p->tinyb.n;
   return COLA_OK;
 }
 
@@ -1023,7 +1083,7 @@ cola_status cola_plan_set_tuning(cola_plan p, const cola_plan_tuning *t) {
       t->blocks_per_cu < 0 || t->blocks_per_cu > 64 || t->blocks_per_cu_short < 0 || t->blocks_per_cu_short > 64 ||
       (t->xcd_remap != 0 && t->xcd_remap != 1) ||
       (t->split_remote != 0 && t->split_remote != 1) || (t->unroll != 0 && t->unroll != 8 && t->unroll != 16) ||
-      (t->panels != 0 && t->panels != 1))
+      (t->panels != 0 && t->panels != 1) || (t->tiny != 0 && t->tiny != 1))
     return fail(COLA_ERR_INVALID, "plan_set_tuning: parameter out of range");
   p->tune = *t;
   return COLA_OK;
@@ -1167,9 +1227,9 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
     if (verbose)
       fprintf(stderr,
               "[autotune %s] tile=%d rpw_short=%d rpw_long=%d blocks_per_cu=%d/%d xcd=%d split=%d unroll=%d "
-              "panels=%d: %.4f ms\n",
+              "panels=%d tiny=%d: %.4f ms\n",
               tag, t.col_tile, t.rpw_short, t.rpw_long, t.blocks_per_cu, t.blocks_per_cu_short, t.xcd_remap,
-              t.split_remote, t.unroll, t.panels, ms);
+              t.split_remote, t.unroll, t.panels, t.tiny, ms);
     if (seen.size() == 1 || ms < bms * 0.99f) {
       bms = ms;
       best = t;
@@ -1240,6 +1300,12 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
       t.unroll = u;
       cands.push_back(t);
     }
+    if (p->tinyb.n > 0)
+      for (int x : {0, 1}) {
+        cola_plan_tuning t = best;
+        t.tiny = x;
+        cands.push_back(t);
+      }
     if (p->panels > 1)
       for (int x : {0, 1}) {
         cola_plan_tuning t = best;

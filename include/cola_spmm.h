@@ -371,16 +371,17 @@ typedef struct {
   int64_t num_short_items, num_long_items;
   int64_t hot_nnz;       /* entries of the first (hot) pass; nnz if one pass */
   int64_t num_cold_items;/* items of the second pass                         */
+  int64_t num_tiny_bundles; /* bundles of rows with at most 4 entries        */
 } cola_plan_info;
 cola_status cola_plan_get_info(cola_plan plan, cola_plan_info *info);
 
 /* Launch parameters of the plan's kernels (the autotuner's search space).
  * A new plan with at least 2^22 entries starts with {-2, 0, 4, 2, 1, 0, 8,
- * 0, 4}: 256-byte column tiles, 4 rows per wave on long rows, 2 blocks per
- * CU (4 for the short-row bin), XCD remap, 8 loads in flight (the tuned
- * schedule of a partition of the community-ordered 2^22-node R-MAT graph);
- * a smaller one with {0, 0, 1, 8, 1, 0, 8, 0, 0}: all columns in one pass,
- * one row per wave on long rows, 8 blocks per CU. */
+ * 0, 4, 1}: 256-byte column tiles, 4 rows per wave on long rows, 2 blocks per
+ * CU (4 for the short-row bin), XCD remap, 8 loads in flight, tiny-row
+ * bundles (the tuned schedule of a partition of the community-ordered
+ * 2^22-node R-MAT graph); a smaller one with {0, 0, 1, 8, 1, 0, 8, 0, 0, 1}:
+ * all columns in one pass, one row per wave on long rows, 8 blocks per CU. */
 typedef struct {
   int32_t col_tile;      /* dense columns per pass over B (0 = all of dim;
                             -k, k = 1..8: k x 128 bytes of the storage
@@ -404,6 +405,12 @@ typedef struct {
                             (0 = blocks_per_cu): short rows are chains of
                             dependent loads and want more waves in flight
                             than long rows                                  */
+  int32_t tiny;          /* 1: rows of at most 4 entries run as bundles of
+                            floor(8/L) rows of L entries per lane group (one
+                            batch of row loads, no padding slots), when the
+                            schedule is one pass over 16-byte row vectors and
+                            a lane group spans at least 8 lanes; 0: those
+                            rows stay in the short-row bin                  */
 } cola_plan_tuning;
 cola_status cola_plan_set_tuning(cola_plan plan, const cola_plan_tuning *t);
 cola_status cola_plan_get_tuning(cola_plan plan, cola_plan_tuning *t);

@@ -181,3 +181,42 @@ def test_all_remote_partition_without_B_local():
     torch.cuda.synchronize()
     ref = vals[:, None].astype(np.float64) * Bh[cols].astype(np.float64)
     assert rel_fro(C.cpu().numpy(), ref) <= FP32_TOL
+
+
+@pytest.mark.parametrize("dtype", [cola.F32, cola.BF16, cola.F16, cola.FP8_E4M3], ids=["f32", "bf16", "f16", "fp8"])
+@pytest.mark.parametrize("dim", [32, 64, 128, 256])
+def test_tiny_row_bundles(dtype, dim):
+    """Rows of 1-4 entries run as bundles of floor(8/L) rows per lane group:
+    bitwise equal to the short-row bin (one row per lane group, same entry
+    order) and within the storage bound of FP64, in overwrite and accumulate
+    mode, with and without XCD ranges, over whole rows and column tiles."""
+    g = cola.Graph.rmat(14, 6, seed=12, reorder=True)  # many rows of 1-4 entries
+    rp, ci, va = g.arrays()
+    nGPU, rank = 2, 0
+    part = g.partition(nGPU, rank)
+    P, M = part.info()["nodesPerPE"], part.info()["num_rows"]
+    plan = cola.Plan(part, row_order=1)
+    assert plan.info["num_tiny_bundles"] > 0
+    Bf = torch.from_numpy(np.random.default_rng(dim).uniform(-1, 1, (P * nGPU, dim)).astype(np.float32)).to(DEV)
+    Bq, sc = (Bf, None) if dtype == cola.F32 else cola.quantize_rows(Bf, dtype)
+    tdt = {cola.F32: torch.float32, cola.BF16: torch.uint16, cola.F16: torch.uint16, cola.FP8_E4M3: torch.uint8}[dtype]
+    packed = torch.empty((max(plan.U_r, 1), dim), dtype=tdt, device=DEV)
+    psc = torch.empty(max(plan.U_r, 1), dtype=torch.float32, device=DEV) if dtype == cola.FP8_E4M3 else None
+    plan.pack(Bq, packed, dtype=dtype, scales_global=sc, packed_scales=psc)
+    sl = None if sc is None else sc[:P]
+    ref = orc.spmm_f64(rp, ci, va, 0, M, Bf.cpu().numpy()[:g.n])
+    tol = {cola.F32: 1e-5, cola.BF16: 4e-3, cola.F16: 5e-4, cola.FP8_E4M3: 5e-2}[dtype]
+    for tile in (0, -1):
+        for xcd in (0, 1):
+            outs = {}
+            for tiny in (0, 1):
+                plan.tuning = dict(col_tile=tile, rpw_short=0, xcd_remap=xcd, split_remote=0, tiny=tiny)
+                for flags, init in ((cola.OVERWRITE, 5.0), (cola.ACCUMULATE, 0.25)):
+                    C = torch.full((P, dim), init, dtype=torch.float32, device=DEV)
+                    plan.spmm(C, Bq[:P], packed, dtype=dtype, flags=flags, scales_local=sl, packed_scales=psc)
+                    torch.cuda.synchronize()
+                    outs[(tiny, flags)] = C.cpu().numpy()
+            for flags, init in ((cola.OVERWRITE, 0.0), (cola.ACCUMULATE, 0.25)):
+                assert np.array_equal(outs[(0, flags)], outs[(1, flags)]), (tile, xcd, flags)
+                got = outs[(1, flags)][:M] - init
+                assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= tol
