@@ -52,6 +52,7 @@ struct BatchSeg {
   int32_t n_items, dim, rpw, n_tiles, tile_w, g_log2, task_begin, task_end;
   int32_t n_grp;  // row groups of rpw items: tasks = n_grp x n_tiles
   uint64_t abase; // common base of B and Bp for 32-bit row offsets (RowArgs::abase), 0: none
+  uint64_t zrow;  // the zero row (RowArgs::zrow)
 };
 
 struct HubSeg {
@@ -112,6 +113,7 @@ __global__ __launch_bounds__(256) void batch_rows_kernel(const BatchSeg *__restr
     a.xcd_remap = 0;
     a.phase = 0;
     a.abase = sg.abase;
+    a.zrow = sg.zrow;
     const int it0 = grp * sg.rpw;
     // The lane-group width is wave-uniform (one segment per task): each case
     // is the plan kernel's step with that width compiled in.
@@ -262,13 +264,16 @@ static cola_status append_problem(const cola_batch_problem &pr, const cola_batch
     while (r * 2 <= req && r * 2 <= S) r *= 2;
     return r;
   };
-  // Common base of the problem's two row tables when every addressable row
-  // (local rows below nodesPerPE, packed rows below U_r, all columns) lies
-  // within 2^36 bytes of it on 16-byte boundaries (RowArgs::abase).
+  // Common base of the problem's two row tables and the zero row when every
+  // addressable row (local rows below nodesPerPE, packed rows below U_r, all
+  // columns) lies within 2^36 bytes of it on 16-byte boundaries
+  // (RowArgs::abase).
+  uint64_t zrow = 0;
+  if (cola_status zst = zero_row_address(&zrow)) return zst;
   uint64_t abase = 0;
   {
     const uint64_t eb = dtype_bytes(dtype);
-    uint64_t lo = UINT64_MAX, hi = 0;
+    uint64_t lo = zrow, hi = zrow + cola_dev::kZeroRowBytes;
     bool ok = true;
     auto span = [&](const void *base, int64_t rows, int64_t ld) {
       if (!base || rows <= 0) return;
@@ -279,7 +284,7 @@ static cola_status append_problem(const cola_batch_problem &pr, const cola_batch
     };
     span(pr.B, p->nodesPerPE, pr.ld_B);
     span(pr.packed, p->remote_rows, pr.ld_packed);
-    if (ok && lo != UINT64_MAX && hi - lo < (uint64_t(1) << 36)) abase = lo;
+    if (ok && hi - lo < (uint64_t(1) << 36)) abase = lo;
   }
   const PlanBin *bins[3] = {&p->longb, &p->shortb, (flags & COLA_OVERWRITE) ? &p->emptyb : nullptr};
   const int rpws[3] = {clamp(t.rpw_long > 0 ? t.rpw_long : 1), clamp(t.rpw_short > 0 ? t.rpw_short : S), S};
@@ -313,6 +318,7 @@ static cola_status append_problem(const cola_batch_problem &pr, const cola_batch
     if (*task > INT32_MAX - (1 << 24)) return fail(COLA_ERR_INVALID, "batch: too many tasks");
     s.task_end = (int32_t)*task;
     s.abase = abase;
+    s.zrow = zrow;
     segs->push_back(s);
     costs->push_back(SegCost{&bin->cost, s.task_begin, s.task_end, s.n_grp, s.rpw, s.tile_w, s.dim, s.n_items});
   }

@@ -16,6 +16,8 @@ int device_cu_count();
 size_t dtype_bytes(int dt);
 int vec_elems(int dt);  // elements per 16-byte lane vector
 bool aligned(const void *p, size_t bytes);
+// Device address of the row kernels' zero row on the current device.
+cola_status zero_row_address(uint64_t *out);
 
 template <typename T>
 cola_status upload(const std::vector<T> &v, T **d) {

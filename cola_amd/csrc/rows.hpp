@@ -120,7 +120,6 @@ struct Vec<COLA_F32, 4> {
   using Raw = float4;
   static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const float4 *>(p); }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global_f4(a); }
-  static __device__ __forceinline__ Raw zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[4]) {
     x[0] = r.x; x[1] = r.y; x[2] = r.z; x[3] = r.w;
   }
@@ -131,7 +130,6 @@ struct Vec<COLA_F32, 1> {
   using Raw = float;
   static __device__ __forceinline__ Raw load(const T *p) { return *p; }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global<float>(a); }
-  static __device__ __forceinline__ Raw zero() { return 0.f; }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = r; }
 };
 template <int DT>
@@ -140,7 +138,6 @@ struct Vec16 {
   using Raw = uint4;
   static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const uint4 *>(p); }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global_u4(a); }
-  static __device__ __forceinline__ Raw zero() { return make_uint4(0u, 0u, 0u, 0u); }
   static __device__ __forceinline__ float cvt(uint32_t h) {
     if constexpr (DT == COLA_BF16) return bf16_to_f32(h);
     else return f16_to_f32(h);
@@ -164,7 +161,6 @@ struct Vec16x1 {
   using Raw = uint32_t;
   static __device__ __forceinline__ Raw load(const T *p) { return *p; }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global<uint16_t>(a); }
-  static __device__ __forceinline__ Raw zero() { return 0u; }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = Vec16<DT>::cvt(r); }
 };
 template <>
@@ -177,7 +173,6 @@ struct Vec<COLA_FP8_E4M3, 16> {
   using Raw = uint4;
   static __device__ __forceinline__ Raw load(const T *p) { return *reinterpret_cast<const uint4 *>(p); }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global_u4(a); }
-  static __device__ __forceinline__ Raw zero() { return make_uint4(0u, 0u, 0u, 0u); }
   // gfx950's v_cvt_pk_f32_fp8 decodes OCP E4M3 (checked against the format
   // definition for all 256 codes), two bytes per instruction.
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[16]) {
@@ -199,9 +194,12 @@ struct Vec<COLA_FP8_E4M3, 1> {
   using Raw = uint32_t;
   static __device__ __forceinline__ Raw load(const T *p) { return *p; }
   static __device__ __forceinline__ Raw gload(uint64_t a) { return load_global<uint8_t>(a); }
-  static __device__ __forceinline__ Raw zero() { return 0u; }
   static __device__ __forceinline__ void widen(const Raw &r, float (&x)[1]) { x[0] = fp8e4m3_to_f32(r); }
 };
+
+// Bytes of the zero row: the widest column tile a row kernel reads (256
+// 16-byte lane vectors, the limit of pick_shape).
+inline constexpr int kZeroRowBytes = 4096;
 
 struct RowArgs {
   const int4 *items;
@@ -230,6 +228,8 @@ struct RowArgs {
                   // 32-bit multiple of 16 bytes (the host checked the span
                   // and alignment), so one word per entry is broadcast
                   // instead of two; 0: full 64-bit addresses
+  uint64_t zrow;  // kZeroRowBytes of zeros (with abase: inside its span), the
+                  // row a staged slot past an item's end reads
 };
 
 // Two forms of the staging (PIPE): false turns the next chunk's staged
@@ -287,9 +287,11 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
   // address arithmetic thus runs once per entry rather than once per
   // broadcast slot.  FP8 rows also take their scale there, multiplied into
   // the value when the chunk becomes current (rotate).  A lane
-  // whose entry lies past the item's end stages the last entry with value 0,
-  // so every broadcast slot holds a valid row address and no load or FMA is
-  // predicated.
+  // whose entry lies past the item's end stages the zero row (a.zrow) with
+  // value 0, so every broadcast slot holds a valid row address, no load or
+  // FMA is predicated, and an Inf or NaN in a B row reaches a sum only
+  // through a stored entry (the reference multiplies exactly the entries
+  // of [rowPtr[v], rowPtr[v+1])).
   constexpr int K = U > G ? U / G : 1;  // entries per lane: one batch of U per group fills K*tsize
   const int csize = K * tsize;
   uint32_t mlo[K], mhi[K], mlo_next[K], mhi_next[K];
@@ -309,8 +311,9 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
     for (int k = 0; k < K; ++k) {
       const int c = nc[k];
       const bool loc = c >= 0;
+      const bool in = base + k * tsize + tl < e;
       const T *rp = loc ? Bl + (int64_t)c * a.ldB : Bp + (int64_t)(~c) * a.ldP;
-      const uint64_t p = reinterpret_cast<uint64_t>(rp);
+      const uint64_t p = in ? reinterpret_cast<uint64_t>(rp) : a.zrow;
       if (a.abase) {
         mlo_next[k] = (uint32_t)((p - a.abase) >> 4);
         mhi_next[k] = 0;
@@ -318,8 +321,8 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
         mlo_next[k] = (uint32_t)p;
         mhi_next[k] = (uint32_t)(p >> 32);
       }
-      mv_next[k] = base + k * tsize + tl < e ? nx[k] : 0.f;
-      if constexpr (DT == COLA_FP8_E4M3) msc_next[k] = gld(loc ? a.sl + c : a.sp + ~c);
+      mv_next[k] = in ? nx[k] : 0.f;
+      if constexpr (DT == COLA_FP8_E4M3) msc_next[k] = in ? gld(loc ? a.sl + c : a.sp + ~c) : 1.f;
     }
   };
   auto rotate = [&]() {
@@ -408,21 +411,6 @@ __device__ __forceinline__ void row_step(const RowArgs &a, int it0, int ie, int 
         // scheduler interleaves loads with the FP8 / 16-bit widening to save
         // registers and leaves one or two loads in flight instead of U.
         __builtin_amdgcn_sched_barrier(0);
-      }
-      // Slots past the item's end hold the last entry's row with value 0;
-      // their rows are replaced by zeros before use, so an Inf or NaN in
-      // that B row cannot turn 0 * x into NaN (only stored entries are
-      // multiplied, as in the reference's loop over [rowPtr[v],
-      // rowPtr[v+1])).  Only a team's last batch can have such slots: the
-      // branch is skipped when no lane of the wave is in one.
-      if (j + gpt * U > n) {
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int pos = K > 1 ? (u / G) * tsize + (u % G) * gpt + tsub : j + u * gpt + tsub;
-          if (pos >= n)
-#pragma unroll
-            for (int k = 0; k < NV; ++k) raw[u][k] = V::zero();
-        }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u)
@@ -546,8 +534,9 @@ __device__ __forceinline__ void tiny_bundle(const RowArgs &a, const int32_t *__r
   const T *__restrict__ Bl = static_cast<const T *>(a.Bl);
   const T *__restrict__ Bp = static_cast<const T *>(a.Bp);
   const int b = item.y, n = has ? item.z - item.y : 0;  // n = rows * L <= N
-  // Stage: lane gl < n takes entry b + gl; the others re-read entry b with
-  // value 0 so that every broadcast slot holds a valid row address.
+  // Stage: lane gl < n takes entry b + gl; the others stage the zero row
+  // with value 0, so every broadcast slot holds a valid row address and
+  // only stored entries meet an Inf or NaN of B.
   uint32_t mlo = 0, mhi = 0;
   float mv = 0.f;
   if (has) {
@@ -555,16 +544,18 @@ __device__ __forceinline__ void tiny_bundle(const RowArgs &a, const int32_t *__r
     const int c = gld(a.cols + k);
     const float x = gld(a.vals + k);
     const bool loc = c >= 0;
+    const bool in = gl < n;
     const T *rp = loc ? Bl + (int64_t)c * a.ldB : Bp + (int64_t)(~c) * a.ldP;
-    const uint64_t p = reinterpret_cast<uint64_t>(rp);
+    const uint64_t p = in ? reinterpret_cast<uint64_t>(rp) : a.zrow;
     if (a.abase) {
       mlo = (uint32_t)((p - a.abase) >> 4);
     } else {
       mlo = (uint32_t)p;
       mhi = (uint32_t)(p >> 32);
     }
-    mv = gl < n ? x : 0.f;
-    if constexpr (DT == COLA_FP8_E4M3) mv *= gld(loc ? a.sl + c : a.sp + ~c);
+    mv = in ? x : 0.f;
+    if constexpr (DT == COLA_FP8_E4M3)
+      if (in) mv *= gld(loc ? a.sl + c : a.sp + ~c);
   }
   if (n == 0) return;  // past the XCD range's last bundle (uniform over the group)
   float av[N];
@@ -596,13 +587,6 @@ __device__ __forceinline__ void tiny_bundle(const RowArgs &a, const int32_t *__r
   const int rows = n / L;
   const int rid = gl < rows ? gld(rowtab + item.x + gl) : 0;
   __builtin_amdgcn_sched_barrier(0);
-  // Slots past the bundle's entries hold a real row with value 0: zero them
-  // so an Inf or NaN there cannot reach a sum.
-  if (n < N) {
-#pragma unroll
-    for (int u = 0; u < N; ++u)
-      if (u >= n) raw[u] = V::zero();
-  }
   float acc[R][EPL];
 #pragma unroll
   for (int j = 0; j < R; ++j)
@@ -634,11 +618,33 @@ __device__ __forceinline__ void tiny_bundle(const RowArgs &a, const int32_t *__r
   }
 }
 
+// One step of a wave over tiny-row bundles it0 .. it0+S-1 (those below ie),
+// one per lane group.  Shared by tiny_rows_kernel and the batched kernel.
+template <int DT, int EPL, int G>
+__device__ __forceinline__ void tiny_step(const RowArgs &a, const int32_t *__restrict__ rowtab, int it0, int ie) {
+  using T = typename Vec<DT, EPL>::T;
+  const int lane = threadIdx.x & 63;
+  const int sub = lane / G, gl = lane % G, tbase = sub * G;
+  const int col = gl * EPL;
+  const int lcol = col < a.dim ? col : 0;
+  const uint64_t lbase = a.abase + (uint64_t)lcol * sizeof(T);
+  const int it = it0 + sub;
+  const bool has = it < ie;
+  const int4 item = has ? gld(a.items + it) : make_int4(0, 0, 0, 4);
+  // The length is uniform over a lane group; groups of one wave differ only
+  // where the bundles of one length end and the next begin.
+  switch (item.w) {
+    case 1: tiny_bundle<DT, EPL, G, 1>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
+    case 2: tiny_bundle<DT, EPL, G, 2>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
+    case 3: tiny_bundle<DT, EPL, G, 3>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
+    default: tiny_bundle<DT, EPL, G, 4>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
+  }
+}
+
 // Tiny-row bundles (plan items (row offset, begin, end, L)): one bundle per
 // lane group per step, items walked like plan_rows_kernel's (XCD ranges).
 template <int DT, int EPL, int G>
 __global__ __launch_bounds__(256) void tiny_rows_kernel(const RowArgs a, const int32_t *__restrict__ rowtab) {
-  using T = typename Vec<DT, EPL>::T;
   constexpr int S = 64 / G;
   const int wpb = blockDim.x >> 6;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -655,25 +661,8 @@ __global__ __launch_bounds__(256) void tiny_rows_kernel(const RowArgs a, const i
     ib = a.xcd_begin[0];
     ie = a.xcd_begin[8];
   }
-  const int lane = threadIdx.x & 63;
-  const int sub = lane / G, gl = lane % G, tbase = sub * G;
-  const int col = gl * EPL;
-  const int lcol = col < a.dim ? col : 0;
-  const uint64_t lbase = a.abase + (uint64_t)lcol * sizeof(T);
   const int wave = blk * wpb + wid, nwave = nblk * wpb;
-  for (int it0 = ib + wave * S; it0 < ie; it0 += nwave * S) {
-    const int it = it0 + sub;
-    const bool has = it < ie;
-    const int4 item = has ? gld(a.items + it) : make_int4(0, 0, 0, 4);
-    // The length is uniform over a lane group; groups of one wave differ only
-    // where the bundles of one length end and the next begin.
-    switch (item.w) {
-      case 1: tiny_bundle<DT, EPL, G, 1>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
-      case 2: tiny_bundle<DT, EPL, G, 2>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
-      case 3: tiny_bundle<DT, EPL, G, 3>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
-      default: tiny_bundle<DT, EPL, G, 4>(a, rowtab, item, has, gl, tbase, lbase, lcol); break;
-    }
-  }
+  for (int it0 = ib + wave * S; it0 < ie; it0 += nwave * S) tiny_step<DT, EPL, G>(a, rowtab, it0, ie);
 }
 
 // Lanes per row group and vectors per lane for a dim.  vec_ok: dim, leading

@@ -78,6 +78,30 @@ size_t dtype_bytes(int dt) {
 }
 
 int vec_elems(int dt) { return dt == COLA_F32 ? 4 : (dt == COLA_FP8_E4M3 ? 16 : 8); }
+
+// The row kernels' zero row (RowArgs::zrow): a static device array, so zero
+// from load and never written, which every GPU holds for the life of the
+// process at its own address, looked up once per device.
+__device__ __align__(256) uint8_t zero_row[cola_dev::kZeroRowBytes];
+static std::atomic<uint64_t> g_zero_row_dev[64] = {};
+
+cola_status zero_row_address(uint64_t *out) {
+  int dev = -1;
+  cola_status st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (st) return st;
+  if (dev < 0 || dev >= 64) return fail(COLA_ERR_UNSUPPORTED, "device id above 63");
+  uint64_t a = g_zero_row_dev[dev].load(std::memory_order_relaxed);
+  if (!a) {
+    void *ptr = nullptr;
+    if ((st = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(zero_row)), "hipGetSymbolAddress(zero_row)")))
+      return st;
+    if (!ptr) return fail(COLA_ERR_HIP, "hipGetSymbolAddress(zero_row) returned NULL");
+    a = reinterpret_cast<uint64_t>(ptr);
+    g_zero_row_dev[dev].store(a, std::memory_order_relaxed);
+  }
+  *out = a;
+  return COLA_OK;
+}
 }  // namespace cola_devhost
 
 using cola::fail;
@@ -487,9 +511,11 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
   // below nodesPerPE, packed rows below U_r) lies within 2^36 bytes of it
   // and starts on a 16-byte boundary, so the row kernel can broadcast a
   // 32-bit offset in 16-byte units instead of a 64-bit address.
+  uint64_t zrow = 0;
+  if (cola_status zst = zero_row_address(&zrow)) return zst;
   auto table_base = [&](int c0, int w) -> uint64_t {
     if (!vec_ok) return 0;
-    uint64_t lo = UINT64_MAX, hi = 0;
+    uint64_t lo = zrow, hi = zrow + cola_dev::kZeroRowBytes;
     auto span = [&](const void *base, int64_t rows, int64_t ld) {
       if (!base || rows <= 0) return;
       const uint64_t b = (uint64_t)(uintptr_t)base + (uint64_t)c0 * eb;
@@ -498,7 +524,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
     };
     span(c.Bl, p->nodesPerPE, c.ldB);
     span(c.Bp, p->remote_rows, c.ldP);
-    if (lo == UINT64_MAX || lo % 16 != 0 || hi - lo >= (uint64_t(1) << 36)) return 0;
+    if (lo % 16 != 0 || hi - lo >= (uint64_t(1) << 36)) return 0;
     return lo;
   };
   if (!accumulate && p->num_empty > 0) {
@@ -548,6 +574,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
         a.xcd_remap = t.xcd_remap ? 1 : 0;
         a.phase = 0;
         a.abase = table_base(c0, w);
+        a.zrow = zrow;
         const int64_t waves = (n + a.rpw - 1) / a.rpw;
         int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bpc);
         blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
@@ -610,6 +637,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
         a.xcd_remap = t.xcd_remap ? 1 : 0;
         a.phase = phase;
         a.abase = table_base(c0, w);
+        a.zrow = zrow;
         const int64_t waves = (bin->n + a.rpw - 1) / a.rpw;
         const int bin_bpc = bi == 1 && t.blocks_per_cu_short > 0 ? t.blocks_per_cu_short : bpc;
         int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bin_bpc);
@@ -640,6 +668,7 @@ cola_status run_plan(const cola_plan_s *p, const SpmmCall &c, const cola_plan_tu
         a.xcd_remap = t.xcd_remap ? 1 : 0;
         a.phase = 0;
         a.abase = table_base(c0, w);
+        a.zrow = zrow;
         const int64_t waves = (p->tinyb.n + S - 1) / S;
         const int bin_bpc = t.blocks_per_cu_short > 0 ? t.blocks_per_cu_short : bpc;
         int64_t blocks = std::min<int64_t>((waves + 3) / 4, (int64_t)device_cu_count() * bin_bpc);
@@ -900,7 +929,7 @@ static cola_status plan_from_partition(const cola::Partition &part, const cola_p
   // blocks per CU and each XCD on its own item range; small ones want every
   // column in one pass and many waves.
   p->tune = hp.nnz >= (int64_t(1) << 22) ? cola_plan_tuning{-2, 0, 4, 2, 1, 0, 8, 0, 4, 1}
-                                         : cola_plan_tuning{0, 0, 1, 8, 1, 0, 8, 0, 0, 1};
+                                         : cola_plan_tuning{0, 0, 1, 8, 1, 0, 8, 0, 0, 0};
   p->num_empty = (int64_t)hp.empty_rows.size();
   p->num_hubs = (int64_t)hp.hub_rows.size();
   p->num_chunks = hp.hub_chunk_ptr.back();
@@ -950,6 +979,8 @@ static cola_status plan_from_partition(const cola::Partition &part, const cola_p
   };
   p->shortb.cost = cost_prefix(is);
   p->longb.cost = cost_prefix(il);
+  p->tinyb.cost = cost_prefix(to_int4(hp.items_tiny));
+  p->shortb_nt.cost = cost_prefix(to_int4(hp.items_short_nt));
   {
     std::vector<int4> ie(hp.empty_rows.size());
     for (size_t i = 0; i < ie.size(); ++i) ie[i] = make_int4(hp.empty_rows[i], 0, 0, 0);

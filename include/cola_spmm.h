@@ -380,8 +380,9 @@ cola_status cola_plan_get_info(cola_plan plan, cola_plan_info *info);
  * 0, 4, 1}: 256-byte column tiles, 4 rows per wave on long rows, 2 blocks per
  * CU (4 for the short-row bin), XCD remap, 8 loads in flight, tiny-row
  * bundles (the tuned schedule of a partition of the community-ordered
- * 2^22-node R-MAT graph); a smaller one with {0, 0, 1, 8, 1, 0, 8, 0, 0, 1}:
- * all columns in one pass, one row per wave on long rows, 8 blocks per CU. */
+ * 2^22-node R-MAT graph); a smaller one with {0, 0, 1, 8, 1, 0, 8, 0, 0, 0}:
+ * all columns in one pass, one row per wave on long rows, 8 blocks per CU,
+ * no bundles (on a small plan their extra launch costs more than it saves). */
 typedef struct {
   int32_t col_tile;      /* dense columns per pass over B (0 = all of dim;
                             -k, k = 1..8: k x 128 bytes of the storage
