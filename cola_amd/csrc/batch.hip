@@ -71,19 +71,47 @@ struct XcdBounds {
   int32_t v[9];
 };
 
+// Largest block of the persistent kernel: a whole CU's waves (16) in one
+// block, except FP8 with compensated sums, whose 155 VGPRs do not fit 4
+// waves per SIMD (it keeps blocks of 4 waves).
+constexpr int batch_max_threads(int dt, bool kahan) { return dt == COLA_FP8_E4M3 && kahan ? 256 : 1024; }
+
 template <int DT, int EPL, bool KAHAN>
-__global__ __launch_bounds__(256) void batch_rows_kernel(const BatchSeg *__restrict__ segs, int n_segs,
-                                                         const XcdBounds xb, int accumulate) {
+__global__ __launch_bounds__(batch_max_threads(DT, KAHAN)) void batch_rows_kernel(const BatchSeg *__restrict__ segs,
+                                                                                int n_segs, const XcdBounds xb,
+                                                                                int accumulate, uint64_t *stamps) {
   using T = typename Vec<DT, EPL>::T;
+  __shared__ int next_task;
   const int wpb = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63;
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int xcd = blockIdx.x & 7;
   const int blk = blockIdx.x >> 3;
   const int nblk = (gridDim.x - xcd + 7) >> 3;
   const int t0 = xb.v[xcd], t1 = xb.v[xcd + 1];
-  const int wave = blk * wpb + wid, nwave = nblk * wpb;
+  const uint64_t start = stamps ? wall_clock64() : 0;
+  if (threadIdx.x == 0) next_task = wpb;
+  __syncthreads();
+  // The range's blocks take its tasks in rounds of wpb consecutive tasks per
+  // block, the blocks in serpentine order (block j takes the j-th chunk of
+  // even rounds and the j-th from the end of odd ones: along a round tasks
+  // get lighter, items being by decreasing degree).  Within a block the
+  // waves draw the block's k-th task from an LDS counter, each asking for
+  // its next before it works on the current one.  One block holds a CU's
+  // waves: the CU issues from its oldest waves first, so waves with equal
+  // static shares end hundreds of microseconds apart (measured on 64 BF16
+  // problems: mean wave lifetime 89% of the launch), while waves that share
+  // a counter end together (93%).
+  auto task_of = [&](int k) {
+    const int r = k / wpb;
+    const int j = (r & 1) ? nblk - 1 - blk : blk;
+    return t0 + (r * nblk + j) * wpb + (k - r * wpb);
+  };
   int si = 0;
-  for (int t = t0 + wave; t < t1; t += nwave) {
+  int t = task_of(wid);
+  while (t < t1) {
+    int got = 0;
+    if (lane == 0) got = __hip_atomic_fetch_add(&next_task, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     while (si < n_segs - 1 && t >= segs[si].task_end) ++si;
     const BatchSeg &sg = segs[si];
     // Tile-major within a segment: the waves sweep every row group of one
@@ -116,15 +144,25 @@ __global__ __launch_bounds__(256) void batch_rows_kernel(const BatchSeg *__restr
     a.zrow = sg.zrow;
     const int it0 = grp * sg.rpw;
     // The lane-group width is wave-uniform (one segment per task): each case
-    // is the plan kernel's step with that width compiled in.
+    // is the plan kernel's step with that width compiled in.  Two-lane groups
+    // keep 4 loads in flight instead of 8: with 8 each lane stages 4 entries,
+    // and that case alone set the kernel's register count (BF16 with
+    // compensated sums 137 VGPRs, 3 waves per SIMD; with 4 loads 128, 4 waves).
     switch (sg.g_log2) {
-      case 1: cola_dev::row_step<DT, EPL, 2, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
+      case 1: cola_dev::row_step<DT, EPL, 2, 1, 4, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
       case 2: cola_dev::row_step<DT, EPL, 4, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
       case 3: cola_dev::row_step<DT, EPL, 8, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
       case 4: cola_dev::row_step<DT, EPL, 16, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
       case 5: cola_dev::row_step<DT, EPL, 32, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
       default: cola_dev::row_step<DT, EPL, 64, 1, 8, KAHAN, cola_dev::kBatchPipe<DT>>(a, it0, sg.n_items, sg.rpw); break;
     }
+    t = task_of(__builtin_amdgcn_readfirstlane(got));
+  }
+  // Start and end of every wave, for the autotuner's per-XCD timing.
+  if (stamps && lane == 0) {
+    const int64_t w = (int64_t)blockIdx.x * wpb + wid;
+    stamps[2 * w] = start;
+    stamps[2 * w + 1] = wall_clock64();
   }
 }
 
@@ -150,15 +188,37 @@ __global__ __launch_bounds__(256) void batch_hub_reduce_kernel(const HubSeg *__r
   cola_dev::gst(dst, accumulate ? cola_dev::gld(dst) + s : s);
 }
 
+// Grid of the persistent kernel: min(tasks, 4 x blocks_per_cu x CUs) waves
+// (blocks_per_cu keeps its meaning of 4-wave blocks per CU), spread over the
+// CUs in blocks of up to max_threads, a multiple of 8 blocks (one XCD range
+// per blockIdx % 8).
+struct Grid {
+  int blocks, threads;
+};
+static Grid batch_grid(int64_t tasks, int blocks_per_cu, int max_threads) {
+  const int64_t cus = device_cu_count();
+  const int64_t waves = std::max<int64_t>(1, std::min<int64_t>(tasks, cus * 4 * blocks_per_cu));
+  const int64_t wpb = std::min<int64_t>(max_threads / 64, (waves + cus - 1) / cus);
+  int64_t blocks = (waves + wpb - 1) / wpb;
+  blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
+  return Grid{(int)blocks, (int)wpb * 64};
+}
+
+// Most waves one launch can have (the autotuner's stamp buffer).
+static int64_t batch_max_waves() { return (int64_t)device_cu_count() * 4 * 64 + 8 * 16; }
+
 template <int DT, int EPL>
-void launch_batch(const BatchSeg *segs, int n_segs, const XcdBounds &xb, int accumulate, bool kahan, int blocks,
-                  hipStream_t s) {
-  if (kahan)
-    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, true>), dim3(blocks), dim3(256), 0, s, segs, n_segs, xb,
-                       accumulate);
-  else
-    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, false>), dim3(blocks), dim3(256), 0, s, segs, n_segs, xb,
-                       accumulate);
+void launch_batch(const BatchSeg *segs, int n_segs, const XcdBounds &xb, int accumulate, bool kahan, int64_t tasks,
+                  int blocks_per_cu, uint64_t *stamps, hipStream_t s) {
+  if (kahan) {
+    const Grid g = batch_grid(tasks, blocks_per_cu, batch_max_threads(DT, true));
+    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, true>), dim3(g.blocks), dim3(g.threads), 0, s, segs, n_segs, xb,
+                       accumulate, stamps);
+  } else {
+    const Grid g = batch_grid(tasks, blocks_per_cu, batch_max_threads(DT, false));
+    hipLaunchKernelGGL((batch_rows_kernel<DT, EPL, false>), dim3(g.blocks), dim3(g.threads), 0, s, segs, n_segs, xb,
+                       accumulate, stamps);
+  }
 }
 
 // Cost model of a segment for the XCD split: a task (one row group of one
@@ -342,17 +402,23 @@ static cola_status append_problem(const cola_batch_problem &pr, const cola_batch
 // Launches the persistent row kernel over a work list and the hub reduction.
 static cola_status launch_tables(const BatchSeg *d_segs, int n_segs, int total_tasks, const XcdBounds &xb,
                                  const HubSeg *d_hubs, int n_hubs, int64_t hub_threads, int dtype, int flags,
-                                 int blocks_per_cu, hipStream_t s) {
+                                 int blocks_per_cu, hipStream_t s, uint64_t *stamps = nullptr) {
   const int accumulate = (flags & COLA_OVERWRITE) ? 0 : 1;
   const bool kahan = (flags & COLA_KAHAN) != 0;
   if (total_tasks > 0) {
-    int64_t blocks = std::min<int64_t>(((int64_t)total_tasks + 3) / 4, (int64_t)device_cu_count() * blocks_per_cu);
-    blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
     switch (dtype) {
-      case COLA_F32: launch_batch<COLA_F32, 4>(d_segs, n_segs, xb, accumulate, kahan, (int)blocks, s); break;
-      case COLA_BF16: launch_batch<COLA_BF16, 8>(d_segs, n_segs, xb, accumulate, kahan, (int)blocks, s); break;
-      case COLA_F16: launch_batch<COLA_F16, 8>(d_segs, n_segs, xb, accumulate, kahan, (int)blocks, s); break;
-      case COLA_FP8_E4M3: launch_batch<COLA_FP8_E4M3, 16>(d_segs, n_segs, xb, accumulate, kahan, (int)blocks, s); break;
+      case COLA_F32:
+        launch_batch<COLA_F32, 4>(d_segs, n_segs, xb, accumulate, kahan, total_tasks, blocks_per_cu, stamps, s);
+        break;
+      case COLA_BF16:
+        launch_batch<COLA_BF16, 8>(d_segs, n_segs, xb, accumulate, kahan, total_tasks, blocks_per_cu, stamps, s);
+        break;
+      case COLA_F16:
+        launch_batch<COLA_F16, 8>(d_segs, n_segs, xb, accumulate, kahan, total_tasks, blocks_per_cu, stamps, s);
+        break;
+      case COLA_FP8_E4M3:
+        launch_batch<COLA_FP8_E4M3, 16>(d_segs, n_segs, xb, accumulate, kahan, total_tasks, blocks_per_cu, stamps, s);
+        break;
     }
   }
   if (hub_threads > 0) {
@@ -418,9 +484,9 @@ static cola_status build_tables(cola_batch_s *b) {
   return COLA_OK;
 }
 
-static cola_status run_batch(cola_batch_s *b, hipStream_t s) {
+static cola_status run_batch(cola_batch_s *b, hipStream_t s, uint64_t *stamps = nullptr) {
   return launch_tables(b->d_segs, b->n_segs, b->total_tasks, b->xb, b->d_hubs, b->n_hubs, b->hub_threads, b->dtype,
-                       b->flags, b->blocks_per_cu, s);
+                       b->flags, b->blocks_per_cu, s, stamps);
 }
 
 static void free_batch(cola_batch_s *b) {
@@ -637,35 +703,44 @@ cola_status cola_batch_autotune(cola_batch b, int iters, void *stream, float *be
     }
     b->blocks_per_cu = best_bpc;
   }
-  // Re-cut the XCD ranges by measured time: each XCD's range is timed alone
-  // (the other XCDs' ranges empty), its time per unit of modelled cost is
-  // spread over its tasks, and the ranges are cut again at equal shares of
-  // that measured time; two rounds, kept only if the whole batch runs faster.
+  // Re-cut the XCD ranges by measured time: launches record every wave's
+  // start and end, each XCD's time (its last wave's end after the launch's
+  // first start, mean of 3 launches) is spread over its range's modelled
+  // cost, and the ranges are cut again at equal shares of that time; three
+  // rounds, each kept only if the whole batch runs faster.
+  uint64_t *d_stamps = nullptr;
+  if (!st && b->n_segs > 0 && b->total_tasks > 0)
+    st = hip_check(hipMalloc((void **)&d_stamps, 2 * batch_max_waves() * sizeof(uint64_t)), "hipMalloc stamps");
   if (!st && b->n_segs > 0 && b->total_tasks > 0) {
-    auto time_rows = [&](const XcdBounds &xb, float *ms) -> cola_status {
-      auto go = [&]() {
-        int64_t blocks = std::min<int64_t>(((int64_t)b->total_tasks + 3) / 4,
-                                           (int64_t)device_cu_count() * b->blocks_per_cu);
-        blocks = std::max<int64_t>(8, (blocks + 7) / 8 * 8);
-        const int acc = (b->flags & COLA_OVERWRITE) ? 0 : 1;
-        const bool kh = (b->flags & COLA_KAHAN) != 0;
-        switch (b->dtype) {
-          case COLA_F32: launch_batch<COLA_F32, 4>(b->d_segs, b->n_segs, xb, acc, kh, (int)blocks, s); break;
-          case COLA_BF16: launch_batch<COLA_BF16, 8>(b->d_segs, b->n_segs, xb, acc, kh, (int)blocks, s); break;
-          case COLA_F16: launch_batch<COLA_F16, 8>(b->d_segs, b->n_segs, xb, acc, kh, (int)blocks, s); break;
-          case COLA_FP8_E4M3: launch_batch<COLA_FP8_E4M3, 16>(b->d_segs, b->n_segs, xb, acc, kh, (int)blocks, s); break;
+    const bool kh = (b->flags & COLA_KAHAN) != 0;
+    const int max_threads = batch_max_threads(b->dtype, kh);
+    std::vector<uint64_t> h_st;
+    int dev = 0, clock_khz = 0;
+    if (!st) st = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (!st) st = hip_check(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, dev), "wall clock rate");
+    if (!st && clock_khz <= 0) clock_khz = 100000;
+    auto xcd_times = [&](double t_x[8]) -> cola_status {
+      const Grid g = batch_grid(b->total_tasks, b->blocks_per_cu, max_threads);
+      const int wpb = g.threads / 64;
+      const int64_t nw = (int64_t)g.blocks * wpb;
+      h_st.resize(2 * nw);
+      for (int x = 0; x < 8; ++x) t_x[x] = 0.0;
+      for (int rep = 0; rep < 3; ++rep) {
+        cola_status r = run_batch(b, s, d_stamps);
+        if (!r) r = hip_check(hipMemcpyAsync(h_st.data(), d_stamps, h_st.size() * sizeof(uint64_t),
+                                             hipMemcpyDeviceToHost, s), "hipMemcpyAsync stamps");
+        if (!r) r = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
+        if (r) return r;
+        uint64_t t0 = UINT64_MAX, end[8] = {};
+        for (int64_t w = 0; w < nw; ++w) t0 = std::min(t0, h_st[2 * w]);
+        for (int64_t w = 0; w < nw; ++w) {
+          const int x = (int)((w / wpb) & 7);
+          end[x] = std::max(end[x], h_st[2 * w + 1]);
         }
-      };
-      go();
-      cola_status r;
-      if ((r = hip_check(hipEventRecord(e0, s), "hipEventRecord"))) return r;
-      for (int i = 0; i < iters; ++i) go();
-      if ((r = hip_check(hipEventRecord(e1, s), "hipEventRecord"))) return r;
-      if ((r = hip_check(hipEventSynchronize(e1), "hipEventSynchronize"))) return r;
-      float t = 0.f;
-      if ((r = hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime"))) return r;
-      *ms = t / iters;
-      return hip_check(hipGetLastError(), "batch XCD timing launch");
+        for (int x = 0; x < 8; ++x) t_x[x] += end[x] > t0 ? (double)(end[x] - t0) : 0.0;
+      }
+      for (int x = 0; x < 8; ++x) t_x[x] /= 3.0 * clock_khz;  // ms
+      return COLA_OK;
     };
     // Modelled cost of the first k tasks of the whole list.
     std::vector<double> seg_before(b->costs.size() + 1, 0.0);
@@ -679,18 +754,12 @@ cola_status cola_batch_autotune(cola_batch b, int iters, void *stream, float *be
     };
     const XcdBounds model_xb = b->xb;
     XcdBounds cur = b->xb, best_xb = b->xb;
-    for (int round = 0; round < 2 && !st; ++round) {
+    for (int round = 0; round < 3 && !st; ++round) {
       double t_x[8];
-      for (int x = 0; x < 8 && !st; ++x) {
-        XcdBounds one{};
-        for (int y = 0; y <= 8; ++y) one.v[y] = y <= x ? cur.v[x] : cur.v[x + 1];
-        float ms = 0.f;
-        st = time_rows(one, &ms);
-        t_x[x] = ms;
-      }
-      if (st) break;
+      b->xb = cur;
+      if ((st = xcd_times(t_x))) break;
       if (verbose) {
-        fprintf(stderr, "[batch autotune] XCD ranges alone (round %d):", round);
+        fprintf(stderr, "[batch autotune] XCD range times (round %d):", round);
         for (int x = 0; x < 8; ++x) fprintf(stderr, " %.3f", t_x[x]);
         fprintf(stderr, " ms\n");
       }
@@ -733,6 +802,7 @@ cola_status cola_batch_autotune(cola_batch b, int iters, void *stream, float *be
     if (verbose && std::memcmp(&best_xb, &model_xb, sizeof(XcdBounds)) != 0)
       fprintf(stderr, "[batch autotune] keeping the XCD ranges cut by measured time\n");
   }
+  (void)hipFree(d_stamps);
   if (!st) st = run_batch(b, s);
   if (!st) st = hip_check(hipStreamSynchronize(s), "hipStreamSynchronize");
   (void)hipEventDestroy(e0);

@@ -507,7 +507,11 @@ cola_status cola_batch_set_tuning(cola_batch b, int problem,
 cola_status cola_batch_get_tuning(cola_batch b, int problem,
                                   cola_batch_tuning *t, int *blocks_per_cu);
 /* Autotuner: coordinate descent over each dim class's column tile and
- * rows-per-wave of both bins, then the grid size, timed with HIP events.
+ * rows-per-wave of both bins, then the grid size, timed with HIP events;
+ * last, the XCD ranges are re-cut by per-XCD times that the kernel records
+ * (kept only if the batch runs faster; a later set_tuning restores the
+ * modelled cut).  blocks_per_cu counts 4-wave blocks: the kernel runs that
+ * many waves per CU, a CU's waves in one block where registers allow.
  * The outputs hold the batch's result on return. */
 cola_status cola_batch_autotune(cola_batch b, int iters, void *stream,
                                 float *best_ms);

@@ -119,6 +119,44 @@ def test_graph_capture_and_replay():
     assert torch.allclose(probs[0]["C"], -eager[0], rtol=0, atol=1e-6)
 
 
+def test_back_to_back_launches_do_every_task_once():
+    """Waves take tasks from per-launch counters that the kernel leaves at
+    zero: five accumulating launches in a row (direct, then from a graph)
+    must add each row's sum exactly five times, which a skipped or repeated
+    task would break (row sums are bitwise the same in every launch)."""
+    probs = make_problems(10, cola.BF16, seed=13)
+    for p in probs:
+        p["C"].zero_()
+    one = cola.Batch(probs, dtype=cola.BF16)
+    one.run()
+    torch.cuda.synchronize()
+    R = [p["C"].cpu().numpy().copy() for p in probs]
+    expect = []
+    for r in R:
+        e = r.copy()
+        for _ in range(4):
+            e = (e + r).astype(np.float32)
+        expect.append(e)
+    acc = cola.Batch(probs, dtype=cola.BF16, flags=0)
+    for p in probs:
+        p["C"].zero_()
+    for _ in range(5):
+        acc.run()
+    torch.cuda.synchronize()
+    for p, e in zip(probs, expect):
+        assert np.array_equal(p["C"].cpu().numpy(), e)
+    s = torch.cuda.Stream()
+    acc.capture(s)
+    for p in probs:
+        p["C"].zero_()
+    torch.cuda.synchronize()
+    for _ in range(5):
+        acc.replay(s)
+    s.synchronize()
+    for p, e in zip(probs, expect):
+        assert np.array_equal(p["C"].cpu().numpy(), e)
+
+
 def test_batch_autotune_keeps_results():
     probs = make_problems(8, cola.BF16, seed=7)
     batch = cola.Batch(probs, dtype=cola.BF16)
