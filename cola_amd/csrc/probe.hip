@@ -90,9 +90,12 @@ extern "C" cola_status cola_probe_gather(cola_plan p, const float *B_local, int6
     return fail(COLA_ERR_INVALID, "probe_gather: matrices must allow 16-byte row vectors");
   if (mode < 0 || mode > 2 || (mode > 0 && table_rows < 1) || (mode == 1 && (table_rows & (table_rows - 1))))
     return fail(COLA_ERR_INVALID, "probe_gather: mode 1 needs a power-of-two table_rows, mode 2 table_rows >= 1");
-  if (mode == 1 && (table_rows > p->nodesPerPE || (p->U_r > 0 && table_rows > p->U_r)))
+  // B_local holds the partition's num_rows rows (local columns lie below
+  // it); the packed buffer holds U_r rows.
+  if (mode == 1 && ((p->U_l > 0 && table_rows > p->num_rows) || (p->U_r > 0 && table_rows > p->U_r)))
     return fail(COLA_ERR_INVALID, "probe_gather: table_rows exceeds a table");
-  if (mode == 2 && table_rows > p->nodesPerPE) return fail(COLA_ERR_INVALID, "probe_gather: table_rows > nodesPerPE");
+  if (mode == 2 && (!B_local || table_rows > p->num_rows))
+    return fail(COLA_ERR_INVALID, "probe_gather: mode 2 reads B_local rows below min(table_rows, num_rows)");
   if (dim > p->max_dim && p->num_chunks > 0) return fail(COLA_ERR_INVALID, "probe_gather: dim above max_dim");
   const int tile = col_tile > 0 && col_tile < dim ? col_tile : dim;
   if (tile % 4 || tile > 256) return fail(COLA_ERR_INVALID, "probe_gather: tile must be a multiple of 4, <= 256");

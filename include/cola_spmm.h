@@ -598,9 +598,11 @@ cola_status cola_error_norms(const float *C, int64_t ld_C, const double *R,
  * multiply-adds into registers and stores each row once, over the plan's
  * long-row then short-row items, one launch per column tile of col_tile
  * columns (0 = all; a multiple of 4, at most 256).  mode 0: the real rows;
- * 1: row indices masked into the first table_rows (a power of two) rows of
- * each table, so the tables stay in L2; 2: entry k reads local row
- * k mod table_rows (the same row loads in address order, no gather).
+ * 1: row indices masked into the first table_rows (a power of two, at most
+ * the partition's rows and U_r) rows of each table, so the tables stay in
+ * L2; 2: entry k reads local row k mod table_rows (table_rows at most the
+ * partition's rows; B_local required): the same row loads in address
+ * order, no gather.
  * Overwrites C (and the hub chunk partials, not reduced).  For timing only. */
 cola_status cola_probe_gather(cola_plan plan, const float *B_local, int64_t ld_B,
                               const float *packed, int64_t ld_packed, float *C,
