@@ -619,7 +619,8 @@ __device__ __forceinline__ void tiny_bundle(const RowArgs &a, const int32_t *__r
 }
 
 // One step of a wave over tiny-row bundles it0 .. it0+S-1 (those below ie),
-// one per lane group.  Shared by tiny_rows_kernel and the batched kernel.
+// one per lane group (tiny_rows_kernel; the batched kernel runs these rows as
+// short-bin items: the bundle's registers would make its other cases spill).
 template <int DT, int EPL, int G>
 __device__ __forceinline__ void tiny_step(const RowArgs &a, const int32_t *__restrict__ rowtab, int it0, int ie) {
   using T = typename Vec<DT, EPL>::T;
