@@ -69,10 +69,12 @@ class Sample:
         return {"rel_fro": float(np.sqrt(out[0] / out[1])), "max_row_rel": float(np.sqrt(out[2]))}
 
 
-def precision_section(cola, torch, orc, log, ctx, args, f32_tuning):
+def precision_section(cola, torch, orc, log, ctx, args, f32_tuning, table=None):
     """BF16 / FP16 / FP8-e4m3 storage of B at dims 64, 128, 256 (FP32
-    accumulation, Kahan on rows above 1024 entries), each autotuned, against
-    FP32 storage at the same dim."""
+    accumulation, Kahan on rows above 1024 entries) against FP32 storage at
+    the same dim, each at the schedule of `table` (the committed schedule
+    table of this workload, made by cola_amd.tune_precision) or, where the
+    table has none, autotuned."""
     plan, P, M, n, rp, ci, va, lb = ctx
     out = {}
     dev = torch.device("cuda:0")
@@ -96,7 +98,14 @@ def precision_section(cola, torch, orc, log, ctx, args, f32_tuning):
             C = torch.empty((P, dim), dtype=torch.float32, device=dev)
             flags = cola.OVERWRITE | (cola.KAHAN if dt != cola.F32 else 0)
             plan.pack(Bq, pq, dtype=dt, scales_global=sc, packed_scales=psc)
-            tq, _ = plan.autotune(C, Bl, pq, dtype=dt, flags=flags, iters=2, scales_local=sl, packed_scales=psc)
+            fixed = (table or {}).get(str(dim), {}).get(name)
+            if fixed is not None:
+                plan.tuning = fixed
+                tq, source = plan.tuning, "profile"
+            else:
+                tq, _ = plan.autotune(C, Bl, pq, dtype=dt, flags=flags, iters=3, scales_local=sl,
+                                      packed_scales=psc)
+                source = "auto"
 
             def step():
                 plan.pack(Bq, pq, dtype=dt, scales_global=sc, packed_scales=psc)
@@ -107,7 +116,7 @@ def precision_section(cola, torch, orc, log, ctx, args, f32_tuning):
             rec = {"ms_per_step": ms,
                    "GBps_alg": bytes_alg(M, plan.nnz, plan.U_l + plan.U_r, dim, cola.DTYPE_BYTES[dt], s_sc) /
                    (ms * 1e-3) / 1e9,
-                   "tuning": tq}
+                   "tuning": tq, "tuning_source": source}
             rec.update(sample.errors(cola, C))
             per[name] = rec
             del Bq, sc, pq, psc, C
@@ -322,6 +331,10 @@ def main():
                          'workload, else autotune), "auto" (autotune before timing), a JSON object of plan '
                          'tuning fields, or @file.json (its "tuning" object)')
     ap.add_argument("--save-tuning", default=None, help="write the FP32 tuning used to this JSON file")
+    ap.add_argument("--precision-tuning", choices=["profile", "auto"], default="profile",
+                    help='schedules of the precision section: "profile" (the committed table '
+                         'profiles/plan_rows_precision_rmat22_tuning.json when made on this workload, else '
+                         'autotune) or "auto"')
     ap.add_argument("--copy-calibrate-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -460,8 +473,18 @@ def main():
         log(f"{args.dtype} errors {extras['%s_errors_sampled_rows' % args.dtype]}")
         del Bh, sample
         if not args.no_precision:
+            # The committed schedule table holds only for the workload it was made on.
+            ptable = None
+            pfile = os.path.join(ROOT, "profiles", "plan_rows_precision_rmat22_tuning.json")
+            if args.precision_tuning == "profile" and os.path.exists(pfile):
+                with open(pfile) as f:
+                    pj = json.load(f)
+                if (pj.get("order") == args.order and pj.get("row_order") == args.row_order
+                        and pj.get("log_n") == args.log_n and pj.get("degree") == args.degree
+                        and pj.get("nparts") == args.nparts and pj.get("rank") == args.rank):
+                    ptable = pj["tunings"]
             extras["precision"] = precision_section(cola, torch, orc, log, (plan, P, M, n, rp, ci, va, lb), args,
-                                                    tuning)
+                                                    tuning, ptable)
             per = extras["precision"].get(str(N), {})
             # Reduced-precision storage against this FP32 step, with its
             # relative Frobenius error against FP64 of the FP32 B.

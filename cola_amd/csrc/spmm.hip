@@ -1270,10 +1270,13 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
   st = consider(best, "start");
   const int ev = vec_elems(dtype);
   // Coupled knobs first: a grid over column tile x grid size x long-row
-  // rows-per-wave x XCD remap x loads in flight.
+  // rows-per-wave x XCD remap x loads in flight.  (Narrow tiles pair with
+  // large grids: the 16-bit forms at dim 256 run fastest at 64-column tiles
+  // and 16 blocks per CU, which coordinate descent from a full-row winner
+  // does not reach one knob at a time.)
   for (int tile : {0, dim / 2, dim / 4}) {
     if (tile != 0 && (tile < ev || tile % ev != 0)) continue;
-    for (int bpc : {1, 2, 4, 8})
+    for (int bpc : {1, 2, 4, 8, 16})
       for (int rl : {1, 2, 4, 8})
         for (int x : {0, 1})
           for (int u : {8, 16}) {
@@ -1351,7 +1354,7 @@ cola_status cola_plan_autotune(cola_plan p, float *C, int64_t ld_C, const void *
   if (!st) {
     std::stable_sort(seen.begin(), seen.end(),
                      [](const auto &x, const auto &y) { return x.first < y.first; });
-    const size_t k = std::min<size_t>(seen.size(), 5);
+    const size_t k = std::min<size_t>(seen.size(), 8);
     std::vector<float> score(k, 3.4e38f);
     for (int round = 0; round < 3 && !st; ++round)
       for (size_t i = 0; i < k && !st; ++i) {

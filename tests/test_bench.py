@@ -41,3 +41,19 @@ def test_bench_json_line_on_a_small_graph():
     c = d["cpu_baseline"]
     assert c["kind"] == "port" and c["value"] > 0 and c["cores"] >= 1
     assert d["config"]["workload"].startswith("rmat14_deg8_community_part0of8")
+
+
+def test_precision_schedule_table_is_complete():
+    """The committed schedule table of the precision section names a full
+    plan tuning for every (dim, storage dtype) the section runs, and the
+    workload fields bench.py checks before using it."""
+    import cola_amd as cola
+    with open(os.path.join(ROOT, "profiles", "plan_rows_precision_rmat22_tuning.json")) as f:
+        d = json.load(f)
+    assert d["order"] == "community" and d["row_order"] == 1
+    assert (d["log_n"], d["degree"], d["nparts"], d["rank"]) == (22, 32, 8, 0)
+    assert d["command"].startswith("python -m cola_amd.tune_precision")
+    for dim in ("64", "128", "256"):
+        for name in ("f32", "bf16", "f16", "fp8_e4m3"):
+            t = d["tunings"][dim][name]
+            assert sorted(t) == sorted(cola.Plan.TUNING_FIELDS), (dim, name)
