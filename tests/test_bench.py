@@ -57,3 +57,23 @@ def test_precision_schedule_table_is_complete():
         for name in ("f32", "bf16", "f16", "fp8_e4m3"):
             t = d["tunings"][dim][name]
             assert sorted(t) == sorted(cola.Plan.TUNING_FIELDS), (dim, name)
+
+
+@pytest.mark.gpu
+def test_tune_precision_writes_a_table(tmp_path):
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("needs a GPU")
+    out = tmp_path / "table.json"
+    run = subprocess.run([sys.executable, "-m", "cola_amd.tune_precision", "--log-n", "12", "--degree", "8",
+                          "--nparts", "2", "--dims", "32", "--dtypes", "f32", "bf16", "--autotunes", "2",
+                          "--rounds", "2", "--reps", "2", "--out", str(out)],
+                         capture_output=True, text=True, timeout=300, cwd=ROOT)
+    assert run.returncode == 0, run.stderr[-2000:]
+    import cola_amd as cola
+    d = json.loads(out.read_text())
+    assert (d["log_n"], d["nparts"], d["rank"], d["order"]) == (12, 2, 0, "community")
+    for name in ("f32", "bf16"):
+        assert sorted(d["tunings"]["32"][name]) == sorted(cola.Plan.TUNING_FIELDS)
+        s = d["step_ms"]["32"][name]
+        assert 1 <= s["candidates"] <= 2 and len(s["median_ms"]) == s["candidates"] and min(s["median_ms"]) > 0
