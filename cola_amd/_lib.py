@@ -144,6 +144,7 @@ SIGNATURES = {
     "cola_server_run_trace": [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_dbl), P_i64],
     "cola_server_destroy": [c_vp],
     "cola_probe_gather": [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_int, c_int, c_int, c_i64, c_int, c_vp],
+    "cola_probe_row_stream": [c_vp, c_i64, c_vp, c_int, c_int, c_int, c_vp, c_i64, c_vp],
     "cola_quantize_rows": [c_vp, c_i64, c_i64, c_int, c_int, c_vp, c_i64, c_vp, c_vp],
     "cola_error_norms": [c_vp, c_i64, c_vp, c_i64, c_i64, c_int, c_vp, c_vp],
 }

@@ -77,7 +77,76 @@ __global__ __launch_bounds__(256) void gather_ceiling_kernel(const int4 *__restr
   }
 }
 
+// The bare gather of a row stream: each 16-lane group takes 8 consecutive
+// entries per step and loads the first 256 bytes of each entry's row (16
+// bytes a lane), all 8 loads in flight, then adds them into registers.  The
+// row of entry k starts at base + row_offs[k] * 16.  The waves stride over
+// the whole stream, or with xcd_ranges block b works on the (b % 8)-th eighth
+// of it, so each XCD walks one contiguous range as the row kernel's XCD item
+// ranges do.  VIA_LDS: the loads go
+// through LDS (global_load_lds_dwordx4 into an 8 KB area per wave, read back
+// after the wave's loads have landed) instead of straight into VGPRs.  Each
+// block stores the sum of its lanes' totals to out[blockIdx.x], which keeps
+// the loads live.
+template <bool VIA_LDS>
+__global__ __launch_bounds__(256) void row_stream_kernel(const uint32_t *__restrict__ row_offs, int64_t n,
+                                                         uint64_t base, int xcd_ranges, float *__restrict__ out) {
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  __shared__ f32x4 slots[4][8][64];
+  __shared__ float wsum[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, gl = lane & 15, grp = lane >> 4;
+  const int xcd = blockIdx.x & 7;
+  const int64_t lo = xcd_ranges ? n * xcd / 8 : 0, hi = xcd_ranges ? n * (xcd + 1) / 8 : n;
+  const int64_t wave = xcd_ranges ? (int64_t)(blockIdx.x >> 3) * 4 + w : (int64_t)blockIdx.x * 4 + w;
+  const int64_t nwave = xcd_ranges ? (int64_t)((gridDim.x - xcd + 7) >> 3) * 4 : (int64_t)gridDim.x * 4;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int64_t b = lo + wave * 32; b < hi; b += nwave * 32) {
+    uint32_t o[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) o[u] = row_offs[min<int64_t>(b + grp * 8 + u, hi - 1)];
+    if constexpr (!VIA_LDS) {
+      f32x4 r[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) r[u] = cola_dev::load_global<f32x4>(base + ((uint64_t)o[u] << 4) + gl * 16);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += r[u];
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        __builtin_amdgcn_global_load_lds(
+            reinterpret_cast<const __attribute__((address_space(1))) void *>(base + ((uint64_t)o[u] << 4) + gl * 16),
+            (__attribute__((address_space(3))) void *)&slots[w][u][0], 16, 0, 0);
+      __builtin_amdgcn_s_waitcnt(0);  // every wait counter at zero: the LDS writes have landed
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc += slots[w][u][lane];
+      __builtin_amdgcn_s_waitcnt(0);  // reads done before the next step refills the slots
+    }
+  }
+  float s = acc.x + acc.y + acc.z + acc.w;
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) wsum[w] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
 }  // namespace
+
+extern "C" cola_status cola_probe_row_stream(const uint32_t *row_offs, int64_t n, const void *base, int via_lds,
+                                             int xcd_ranges, int blocks_per_cu, float *out, int64_t out_len,
+                                             void *stream) {
+  if (n < 0 || (n > 0 && !row_offs) || !base || !out || !aligned(base, 16))
+    return fail(COLA_ERR_INVALID, "probe_row_stream: bad arguments");
+  const int64_t blocks = (int64_t)device_cu_count() * (blocks_per_cu > 0 ? blocks_per_cu : 2);
+  if (out_len < blocks) return fail(COLA_ERR_INVALID, "probe_row_stream: out holds fewer floats than blocks");
+  if (n == 0) return COLA_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const uint64_t b = (uint64_t)(uintptr_t)base;
+  if (via_lds)
+    hipLaunchKernelGGL(row_stream_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, s, row_offs, n, b, xcd_ranges, out);
+  else
+    hipLaunchKernelGGL(row_stream_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, s, row_offs, n, b, xcd_ranges, out);
+  return hip_check(hipGetLastError(), "probe_row_stream launch");
+}
 
 extern "C" cola_status cola_probe_gather(cola_plan p, const float *B_local, int64_t ld_B, const float *packed,
                                          int64_t ld_packed, float *C, int64_t ld_C, int dim, int col_tile, int mode,

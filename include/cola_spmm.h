@@ -610,6 +610,20 @@ cola_status cola_probe_gather(cola_plan plan, const float *B_local, int64_t ld_B
                               int64_t table_rows, int blocks_per_cu,
                               void *stream);
 
+/* No counterpart in the reference.  The bare gather of a row stream: entry
+ * k names a row starting at base + row_offs[k] * 16 bytes; each 16-lane group
+ * loads the first 256 bytes of 8 consecutive entries' rows (16 bytes a lane,
+ * all 8 loads in flight) and adds them up, its waves striding over the
+ * whole stream or (xcd_ranges = 1) each XCD over one contiguous eighth of it,
+ * through VGPRs (via_lds = 0) or
+ * through LDS (via_lds = 1: global_load_lds_dwordx4).  Nothing else is read
+ * or written but one float per block into out (out_len >= CUs x
+ * blocks_per_cu, 0 = 2).  Every row must have 256 readable bytes and base
+ * must be 16-byte aligned.  For timing only. */
+cola_status cola_probe_row_stream(const uint32_t *row_offs, int64_t n, const void *base,
+                                  int via_lds, int xcd_ranges, int blocks_per_cu,
+                                  float *out, int64_t out_len, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

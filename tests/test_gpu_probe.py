@@ -1,8 +1,9 @@
-"""The gather-ceiling measurement kernel (cola_probe_gather) against the FP64
-oracle: mode 0 computes the partition product over the plan's items, mode 1
-the same product with every row index masked into the first table_rows rows
-of each table; mode 2 runs; bad arguments (and tables the buffers do not
-hold) are rejected before any launch."""
+"""The gather-ceiling measurement kernels.  cola_probe_gather against the
+FP64 oracle: mode 0 computes the partition product over the plan's items,
+mode 1 the same product with every row index masked into the first
+table_rows rows of each table; mode 2 runs; bad arguments (and tables the
+buffers do not hold) are rejected before any launch.  cola_probe_row_stream
+(the bare gather of a row stream) reads exactly the rows it is given."""
 import ctypes
 
 import numpy as np
@@ -105,3 +106,40 @@ def test_bad_arguments_are_rejected(case, mode, table_rows, tile, with_B_local):
     C = torch.zeros((c["M"], c["dim"]), dtype=torch.float32, device=DEV)
     assert probe(c, C, mode, table_rows, tile, with_B_local) == _lib.COLA_ERR_INVALID
     assert _lib.lib().cola_last_error()
+
+
+@pytest.mark.parametrize("xcd_ranges", [0, 1], ids=["strided", "xcd_ranges"])
+@pytest.mark.parametrize("via_lds", [0, 1], ids=["vgpr", "lds"])
+def test_row_stream_gather_sums_exactly_the_named_rows(via_lds, xcd_ranges):
+    """cola_probe_row_stream loads the first 256 bytes of every named row once
+    (a stream of whole 32-entry steps, so no entry is re-read at the end):
+    with small-integer rows the per-block sums add up exactly, through VGPRs
+    and through LDS (global_load_lds_dwordx4 lane layout)."""
+    rng = np.random.default_rng(9)
+    rows, width = 3000, 96  # 384-byte rows
+    Bh = rng.integers(0, 4, (rows, width)).astype(np.float32)
+    B = torch.from_numpy(Bh).to(DEV)
+    n = 32 * 64
+    idx = rng.integers(0, rows, n)
+    offs = torch.from_numpy((idx * width * 4 // 16).astype(np.uint32).view(np.int32)).to(DEV)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    out = torch.zeros(cus * 2, dtype=torch.float32, device=DEV)
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    st = _lib.lib().cola_probe_row_stream(ctypes.c_void_p(offs.data_ptr()), n, ctypes.c_void_p(B.data_ptr()),
+                                          via_lds, xcd_ranges, 2, ctypes.c_void_p(out.data_ptr()), out.numel(), sp)
+    assert st == 0, _lib.lib().cola_last_error()
+    torch.cuda.synchronize()
+    assert float(out.double().sum()) == float(Bh[idx, :64].astype(np.float64).sum())
+
+
+def test_row_stream_rejects_bad_arguments():
+    B = torch.zeros((4, 64), dtype=torch.float32, device=DEV)
+    offs = torch.zeros(32, dtype=torch.int32, device=DEV)
+    out = torch.zeros(8, dtype=torch.float32, device=DEV)  # fewer floats than blocks
+    sp = ctypes.c_void_p(0)
+    lib = _lib.lib()
+    assert lib.cola_probe_row_stream(ctypes.c_void_p(offs.data_ptr()), 32, ctypes.c_void_p(B.data_ptr()), 0, 0, 2,
+                                     ctypes.c_void_p(out.data_ptr()), out.numel(), sp) == _lib.COLA_ERR_INVALID
+    big = torch.zeros(1 << 16, dtype=torch.float32, device=DEV)
+    assert lib.cola_probe_row_stream(ctypes.c_void_p(offs.data_ptr()), 32, ctypes.c_void_p(B.data_ptr() + 4), 0, 0,
+                                     2, ctypes.c_void_p(big.data_ptr()), big.numel(), sp) == _lib.COLA_ERR_INVALID
