@@ -5,6 +5,9 @@ Bars: FP32 storage -> relative Frobenius error <= 1e-5 against the FP64 oracle
 conversions) -> bit-exact; reduced-precision storage -> the error bounds
 stated per dtype below.
 """
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -17,6 +20,7 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not torch.cuda.is_available(), reason="needs a GPU")]
 
 FP32_TOL = 1e-5
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def rel_fro(C, R):
@@ -719,6 +723,35 @@ def test_full_size_precision_forms(full_size, dtype, dim):
     assert err_q <= FP32_TOL
     assert err <= PREC_TOL[dtype]
 
+
+
+@pytest.mark.parametrize("dim", [64, 256])
+def test_full_size_f32_other_dims(full_size, dim):
+    """FP32 B at the precision configuration's other dims (64, 256) on the
+    2^22-node partition, at the schedule bench.py's precision sweep uses for
+    it (the committed table, community order) or the default schedule
+    (Cuthill-McKee order): 527 sampled rows within 1e-5 of the FP64 product."""
+    g, part = full_size["g"], full_size["part"]
+    rp, ci, va = g.arrays()
+    info = part.info()
+    P, M = info["nodesPerPE"], info["num_rows"]
+    gen = torch.Generator(device="cuda").manual_seed(70 + dim)
+    B = torch.rand((P * 8, dim), generator=gen, device="cuda", dtype=torch.float32) * 2 - 1
+    plan = cola.Plan(part, row_order=1)
+    if full_size["order"] == "community":
+        with open(os.path.join(ROOT, "profiles", "plan_rows_precision_rmat22_tuning.json")) as f:
+            plan.tuning = json.load(f)["tunings"][str(dim)]["f32"]
+    print(f"case full size f32 order={full_size['order']} dim={dim} tuning={plan.tuning}", flush=True)
+    packed = torch.empty((plan.U_r, dim), dtype=torch.float32, device="cuda")
+    C = torch.full((P, dim), 3.0, dtype=torch.float32, device="cuda")
+    plan.pack(B, packed)
+    plan.spmm(C, B[:P], packed, flags=cola.OVERWRITE)
+    torch.cuda.synchronize()
+    sample = np.unique(np.concatenate([np.arange(0, M, 997), [0, 1, M - 1]]))
+    rp2, ci2, va2, cols = sampled_rows_csr(rp, ci, va, sample)
+    Bf = B[torch.from_numpy(cols.astype(np.int64)).cuda()].cpu().numpy()
+    ref = orc.spmm_f64(rp2, ci2, va2, 0, len(sample), Bf)
+    assert rel_fro(C.cpu().numpy()[sample], ref) <= FP32_TOL
 
 TUNINGS = [dict(col_tile=0, rpw_short=0, rpw_long=1, blocks_per_cu=8, xcd_remap=1, split_remote=0, unroll=8, panels=0),
            dict(col_tile=32, rpw_short=1, rpw_long=2, blocks_per_cu=2, xcd_remap=0, split_remote=1),
