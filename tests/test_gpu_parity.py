@@ -671,15 +671,18 @@ def sampled_rows_csr(rp, ci, va, rows):
     return rp2, inv.astype(np.int32), np.asarray(va, np.float32)[idx], cols
 
 
+@pytest.mark.parametrize("schedule", ["default", "tuned"])
 @pytest.mark.parametrize("dim", [64, 128, 256])
 @pytest.mark.parametrize("dtype", [cola.BF16, cola.F16, cola.FP8_E4M3], ids=["bf16", "f16", "fp8"])
-def test_full_size_precision_forms(full_size, dtype, dim):
+def test_full_size_precision_forms(full_size, dtype, dim, schedule):
     """B stored in BF16 / FP16 / FP8-e4m3 (+ row scales) on the 2^22-node
     partition at each feature dim of the precision configuration (64, 128,
     256), FP32 accumulation with Kahan on rows above 1024 entries: on 527
     sampled rows the result equals the FP64 product of the dequantised B to
     FP32 accuracy (1e-5), and is within the storage format's error of the
-    FP64 product of the FP32 B."""
+    FP64 product of the FP32 B.  Each at the plan's default schedule and at
+    a tuned one: the bench's committed table on the community order, the
+    plan autotuner's pick on the Cuthill-McKee order."""
     g, part = full_size["g"], full_size["part"]
     rp, ci, va = g.arrays()
     info = part.info()
@@ -693,11 +696,22 @@ def test_full_size_precision_forms(full_size, dtype, dim):
           flush=True)
     Bq, sc = cola.quantize_rows(B, dtype)
     plan = cola.Plan(part, row_order=1)
+    tuned = schedule == "tuned" and full_size["order"] == "community"
+    if tuned:
+        # The schedule bench.py's precision sweep runs this form at (its
+        # committed table, made on the community order).
+        with open(os.path.join(ROOT, "profiles", "plan_rows_precision_rmat22_tuning.json")) as f:
+            plan.tuning = json.load(f)["tunings"][str(dim)][cola.DTYPE_NAMES[dtype]]
     tdt = {1: torch.uint8, 2: torch.uint16}[cola.DTYPE_BYTES[dtype]]
     packed = torch.empty((plan.U_r, dim), dtype=tdt, device="cuda")
     psc = torch.empty(plan.U_r, dtype=torch.float32, device="cuda") if dtype == cola.FP8_E4M3 else None
     plan.pack(Bq, packed, dim=dim, dtype=dtype, scales_global=sc, packed_scales=psc)
     C = torch.zeros((P, dim), dtype=torch.float32, device="cuda")
+    if schedule == "tuned" and not tuned:
+        # Cuthill-McKee order: the plan autotuner's pick on these buffers.
+        plan.autotune(C, Bq[:P], packed, dim=dim, dtype=dtype, flags=cola.OVERWRITE | cola.KAHAN, iters=1,
+                      scales_local=None if sc is None else sc[:P], packed_scales=psc)
+        C.zero_()
     plan.spmm(C, Bq[:P], packed, dim=dim, dtype=dtype, flags=cola.OVERWRITE | cola.KAHAN,
               scales_local=None if sc is None else sc[:P], packed_scales=psc)
     torch.cuda.synchronize()
