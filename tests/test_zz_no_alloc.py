@@ -3,8 +3,8 @@ array) against cola_spmm_partition (the plan's own workspace), called
 through the C ABI: the same kernels in the same order, so results are
 bitwise equal.  Two calls in a row, the second on a smaller partition and
 dim; hub-heavy plans, the column-panel schedule, BF16 storage with
-compensated sums, a dim above the plan's max_dim, and a plan too large for
-the array (rejected, C untouched)."""
+compensated sums, a dim above the plan's max_dim, a plan too large for the
+array (rejected, C untouched), and the benchmarked 2^22-node partition."""
 import ctypes
 
 import numpy as np
@@ -109,3 +109,38 @@ def test_noalloc_rejects_plans_beyond_the_array():
     assert b"workspace" in cola.lib().cola_last_error()
     torch.cuda.synchronize()
     assert torch.all(C == 5.0)
+
+
+def test_noalloc_at_the_benchmarked_partition():
+    """Partition 0 of 8 of the 2^22-node community-ordered graph at dim 128
+    with the benchmarked schedule: its hub chunks fit the array (about 10 MB
+    of 16 MiB) and two calls in a row equal cola_spmm_partition bitwise."""
+    import json
+    import os
+    g = cola.Graph.rmat(22, 32, seed=2, reorder="community")
+    part = g.partition(8, 0)
+    P, dim = part.info()["nodesPerPE"], 128
+    plan = cola.Plan(part, row_order=1)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with open(os.path.join(root, "profiles", "plan_rows_f32_rmat22_traffic.json")) as f:
+        plan.tuning = json.load(f)["tuning"]
+    chunks = plan.info["num_hub_chunks"]
+    print(f"case noalloc full size: {chunks} hub chunks", flush=True)
+    assert 0 < chunks * dim * 4 <= SCRATCH_BYTES
+    B = torch.rand((P * 8, dim), generator=torch.Generator(device=DEV).manual_seed(8), device=DEV).mul_(2).sub_(1)
+    packed = torch.empty((plan.U_r, dim), dtype=torch.float32, device=DEV)
+    plan.pack(B, packed)
+    Bl = B[:P]
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def call(name, C):
+        return getattr(cola.lib(), name)(plan.handle, vp(C), C.stride(0), vp(Bl), Bl.stride(0), None, vp(packed),
+                                         packed.stride(0), None, dim, cola.F32, cola.OVERWRITE, sp)
+
+    C_ref = torch.full((P, dim), 3.0, dtype=torch.float32, device=DEV)
+    assert call("cola_spmm_partition", C_ref) == 0, cola.lib().cola_last_error()
+    for _ in range(2):
+        C = torch.full((P, dim), 4.0, dtype=torch.float32, device=DEV)
+        assert call("cola_spmm_partition_noalloc", C) == 0, cola.lib().cola_last_error()
+        torch.cuda.synchronize()
+        assert torch.equal(C, C_ref)
